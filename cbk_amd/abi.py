@@ -1,0 +1,52 @@
+"""Argument tables of the cbk.h C ABI, shared by every ctypes binding.
+
+Each routine maps to its argument kinds in order: 'c' a char option, 'i' an int,
+'T' the scalar type (double for d*, float for s*), 'p' a buffer pointer. The
+lists are the signatures of the reference's include/cbk.h:77-351 (the same
+names and argument order) plus the library's own extensions (include/cbk_ext.h).
+"""
+
+import ctypes
+
+ROUTINES = {
+    # include/cbk.h:77-351 (reference)
+    "geqrf": "ciipipii",
+    "ormqr": "ccciiipippiii",
+    "orgqr": "ciiipipii",
+    "potrf": "ccipiii",
+    "potrs": "cciipipiii",
+    "posv": "cciipipiii",
+    "sytrfnp": "ccipiii",
+    "sytrsnp": "cciipipiii",
+    "sysvnp": "cciipipiii",
+    "trsm": "cccccii" + "T" + "pipiii",
+    "gels": "cciiipipipii",
+}
+
+# per-matrix info variants: the plain argument list plus a trailing device int[nm]
+INFO_ROUTINES = ("potrf", "posv", "sytrfnp", "sysvnp", "gels")
+
+
+def ctype_of(kind, scalar):
+    if kind == "c":
+        return ctypes.c_char
+    if kind == "i":
+        return ctypes.c_int
+    if kind == "p":
+        return ctypes.c_void_p
+    if kind == "T":
+        return ctypes.c_double if scalar == "d" else ctypes.c_float
+    raise ValueError(kind)
+
+
+def bind(lib, name, kinds, scalar, restype=ctypes.c_int):
+    fn = getattr(lib, name)
+    fn.argtypes = [ctype_of(k, scalar) for k in kinds]
+    fn.restype = restype
+    return fn
+
+
+def to_c(kind, value):
+    if kind == "c":
+        return value.encode() if isinstance(value, str) else value
+    return value

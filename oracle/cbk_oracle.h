@@ -1,0 +1,87 @@
+/* cbk_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * The CPU restatement of the reference compact kernels (ivan-pi/cbk), used by
+ * tests/ and by bench.py's CPU-baseline leg as the checker of the HIP library.
+ * It is never linked into, loaded by, or called from the shipped library.
+ *
+ * Same argument lists as include/cbk.h (reference include/cbk.h:77-351), over
+ * HOST pointers, prefixed oracle_. V may be any of 2, 4, 8, 16, 32, 64 (the
+ * reference accepts 2..16, src/cbk.cpp:56-59). Results are bit-identical to the
+ * reference compiled with FMA contraction (oracle/Makefile `ref`), which the
+ * tests check; every multiply-add the reference's compiler contracts is spelt
+ * out here as fma().
+ */
+#ifndef CBK_ORACLE_H
+#define CBK_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int oracle_get_version(void);
+
+#define ORACLE_DECLS(T, p)                                                               \
+    int oracle_##p##geqrf_compact(char layout, int m, int n, T *ap, int ldap, T *taup,   \
+                                  int V, int nm);                                        \
+    int oracle_##p##ormqr_compact(char layout, char side, char trans, int m, int n,      \
+                                  int k, const T *ap, int ldap, const T *taup, T *cp,    \
+                                  int ldcp, int V, int nm);                              \
+    int oracle_##p##orgqr_compact(char layout, int m, int n, int k, T *ap, int ldap,     \
+                                  const T *taup, int V, int nm);                         \
+    int oracle_##p##potrf_compact(char layout, char uplo, int n, T *ap, int ldap, int V, \
+                                  int nm);                                               \
+    int oracle_##p##potrs_compact(char layout, char uplo, int n, int nrhs, const T *ap,  \
+                                  int ldap, T *bp, int ldbp, int V, int nm);             \
+    int oracle_##p##posv_compact(char layout, char uplo, int n, int nrhs, T *ap,         \
+                                 int ldap, T *bp, int ldbp, int V, int nm);              \
+    int oracle_##p##sytrfnp_compact(char layout, char uplo, int n, T *ap, int ldap,      \
+                                    int V, int nm);                                      \
+    int oracle_##p##sytrsnp_compact(char layout, char uplo, int n, int nrhs,             \
+                                    const T *ap, int ldap, T *bp, int ldbp, int V,       \
+                                    int nm);                                             \
+    int oracle_##p##sysvnp_compact(char layout, char uplo, int n, int nrhs, T *ap,       \
+                                   int ldap, T *bp, int ldbp, int V, int nm);            \
+    int oracle_##p##trsm_compact(char layout, char side, char uplo, char transa,         \
+                                 char diag, int m, int n, T alpha, const T *ap,          \
+                                 int ldap, T *bp, int ldbp, int V, int nm);              \
+    int oracle_##p##gels_compact(char layout, char trans, int m, int n, int nrhs, T *ap, \
+                                 int ldap, T *bp, int ldbp, T *taup, int V, int nm);     \
+    /* per-matrix LAPACK info variants (no reference counterpart): the same       \
+     * arithmetic as the plain call plus info[idx] for idx < nm -- potrf/posv: the  \
+     * first j (1-based) whose pivot is <= 0 or NaN; sytrfnp/sysvnp: the first     \
+     * exact zero pivot; gels: the first exact zero diagonal of R. A lane with      \
+     * info > 0 keeps B as LAPACK leaves it: posv/sysvnp and minimum-norm gels      \
+     * unchanged, least-squares gels holding Q^T B. */                             \
+    int oracle_##p##potrf_compact_info(char layout, char uplo, int n, T *ap, int ldap,  \
+                                       int V, int nm, int *info);                       \
+    int oracle_##p##posv_compact_info(char layout, char uplo, int n, int nrhs, T *ap,   \
+                                      int ldap, T *bp, int ldbp, int V, int nm,         \
+                                      int *info);                                       \
+    int oracle_##p##sytrfnp_compact_info(char layout, char uplo, int n, T *ap,          \
+                                         int ldap, int V, int nm, int *info);           \
+    int oracle_##p##sysvnp_compact_info(char layout, char uplo, int n, int nrhs, T *ap, \
+                                        int ldap, T *bp, int ldbp, int V, int nm,       \
+                                        int *info);                                     \
+    int oracle_##p##gels_compact_info(char layout, char trans, int m, int n, int nrhs,  \
+                                      T *ap, int ldap, T *bp, int ldbp, T *taup, int V, \
+                                      int nm, int *info);
+
+ORACLE_DECLS(double, d)
+ORACLE_DECLS(float, s)
+#undef ORACLE_DECLS
+
+/* LAPACK dsposv restated per matrix on the compact layout (no reference
+ * counterpart; LAPACK 3.12 dsposv.f): single-precision Cholesky, refinement with
+ * double residuals, ITERMAX = 30, convergence when ||r||_inf <= ||x||_inf *
+ * ||A||_inf * eps * sqrt(n) for every column, fallback to the double-precision
+ * dposv otherwise. xp receives X; ap keeps A unless the fallback ran, in which
+ * case it holds the double Cholesky factor. iter and info are per matrix. */
+int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
+                          const double *bp, int ldbp, double *xp, int ldxp, int *iter,
+                          int *info, int V, int nm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBK_ORACLE_H */
