@@ -1,0 +1,100 @@
+"""cbk_amd: compact batch kernels on MI355X.
+
+The product is libcbk_amd.so, whose C ABI is include/cbk.h (the drop-in for
+the reference's portable API, over device pointers) and include/cbk_ext.h.
+This module binds that ABI with ctypes so Python callers (the tests, bench.py)
+can pass torch CUDA tensors. It raises when the library is missing: there is
+no CPU fallback.
+"""
+
+import ctypes
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcbk_amd.so")
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded libcbk_amd.so (ctypes.CDLL)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(
+                f"{LIB_PATH} is not built; run `python -m cbk_amd.build` "
+                "(or __graft_entry__.build())")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.cbk_set_stream.argtypes = [ctypes.c_void_p]
+        _lib.cbk_set_stream.restype = ctypes.c_int
+        _lib.cbk_get_stream.argtypes = []
+        _lib.cbk_get_stream.restype = ctypes.c_void_p
+        _lib.cbk_synchronize.argtypes = []
+        _lib.cbk_synchronize.restype = ctypes.c_int
+        _lib.cbk_get_version.argtypes = []
+        _lib.cbk_get_version.restype = ctypes.c_int
+    return _lib
+
+
+_fns = {}
+
+
+def fn(scalar, routine, variant=""):
+    """ctypes function for {scalar}{routine}_compact{variant} with argtypes set.
+    variant: "" (cbk.h) or "_info" (cbk_ext.h, trailing int* info)."""
+    key = (scalar, routine, variant)
+    if key not in _fns:
+        kinds = abi.ROUTINES[routine] + ("p" if variant == "_info" else "")
+        name = f"{scalar}{routine}_compact{variant}"
+        _fns[key] = (abi.bind(lib(), name, kinds, scalar), kinds)
+    return _fns[key]
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return ctypes.c_void_p(x.data_ptr())
+    if isinstance(x, int):
+        return ctypes.c_void_p(x)
+    raise TypeError(f"expected a device tensor or address, got {type(x)}")
+
+
+def call(scalar, routine, *args, info=None, variant=None):
+    """Call a routine with torch CUDA tensors (or raw device addresses) for the
+    buffers; returns the routine's int status. Launches on the current cbk
+    stream (see set_stream)."""
+    if variant is None:
+        variant = "_info" if info is not None else ""
+    f, kinds = fn(scalar, routine, variant)
+    full = list(args) + ([info] if variant == "_info" else [])
+    if len(full) != len(kinds):
+        raise TypeError(f"{scalar}{routine}: expected {len(kinds)} arguments, got {len(full)}")
+    cargs = [_ptr(a) if k == "p" else abi.to_c(k, a) for k, a in zip(kinds, full)]
+    return f(*cargs)
+
+
+def set_stream(stream):
+    """Make `stream` (a torch.cuda.Stream, a raw hipStream_t int, or None) the
+    stream this thread's calls launch on."""
+    if stream is None:
+        h = None
+    elif isinstance(stream, int):
+        h = stream
+    else:
+        h = stream.cuda_stream
+    return lib().cbk_set_stream(h)
+
+
+def synchronize():
+    return lib().cbk_synchronize()
+
+
+def version():
+    return lib().cbk_get_version()

@@ -1,0 +1,62 @@
+/* cbk_internal.h -- launch interface between the host dispatch (cbk_api.cpp)
+ * and the HIP kernels. Not installed; not part of the C ABI. */
+#ifndef CBK_INTERNAL_H
+#define CBK_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace cbk {
+
+/* the stream of the calling host thread (cbk_set_stream) */
+hipStream_t current_stream();
+
+/* One operand of a compact batch: base pointer, leading dimension, layout and
+ * scalars per group (group_stride of the reference, src/cbk_common.hpp:210). */
+template <class T> struct Operand {
+    T *p;
+    int ld;
+    bool row;
+    size_t gstride;
+};
+
+/* ---- generic lane kernels: one lane = one matrix, any V, any layout ---- */
+
+/* potrf / sytrfnp, optionally followed by potrs / sytrsnp on B (nrhs > 0).
+ * ldl selects sytrfnp. factor=false runs the solve alone (potrs/sytrsnp on an
+ * already-factored A). info (may be null): per-matrix code; with info, a
+ * failed matrix skips its solve. */
+template <class T>
+hipError_t launch_chol_lane(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
+                            Operand<T> B, int V, int nm, int *info);
+
+/* trsm (any side/uplo/trans/diag/layout); alpha == 0 zeroes B */
+template <class T>
+hipError_t launch_trsm_lane(bool left, bool up, bool tran, bool unit, int m, int n, T alpha,
+                            Operand<T> A, Operand<T> B, int V, int nm);
+
+template <class T>
+hipError_t launch_geqrf_lane(int m, int n, Operand<T> A, T *tau, int V, int nm);
+
+template <class T>
+hipError_t launch_ormqr_lane(bool left, bool tran, int m, int n, int k, Operand<T> A,
+                             const T *tau, Operand<T> C, int V, int nm);
+
+template <class T>
+hipError_t launch_orgqr_lane(int m, int n, int k, Operand<T> A, const T *tau, int V, int nm);
+
+template <class T>
+hipError_t launch_gels_lane(bool tran, int m, int n, int nrhs, Operand<T> A, Operand<T> B,
+                            T *tau, int V, int nm, int *info);
+
+/* B(0:rows, 0:cols) := 0 in every group */
+template <class T>
+hipError_t launch_zero(int rows, int cols, Operand<T> B, int V, int nm);
+
+/* info[0:nm] := 0 */
+hipError_t launch_clear_info(int *info, int nm);
+
+} /* namespace cbk */
+
+#endif /* CBK_INTERNAL_H */

@@ -1,0 +1,39 @@
+/* dispatch.cpp -- kernel selection (see dispatch.h). */
+
+#include "dispatch.h"
+
+namespace cbk {
+
+template <class T>
+hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
+                        Operand<T> B, int V, int nm, int *info)
+{
+    return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+}
+
+template <class T>
+hipError_t dispatch_trsm(bool left, bool up, bool tran, bool unit, int m, int n, T alpha,
+                         Operand<T> A, Operand<T> B, int V, int nm)
+{
+    return launch_trsm_lane<T>(left, up, tran, unit, m, n, alpha, A, B, V, nm);
+}
+
+template <class T>
+hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operand<T> B,
+                         T *tau, int V, int nm, int *info)
+{
+    return launch_gels_lane<T>(tran, m, n, nrhs, A, B, tau, V, nm, info);
+}
+
+#define CBK_INSTANTIATE_DISPATCH(T)                                                      \
+    template hipError_t dispatch_sym<T>(bool, bool, bool, int, int, Operand<T>,           \
+                                        Operand<T>, int, int, int *);                     \
+    template hipError_t dispatch_trsm<T>(bool, bool, bool, bool, int, int, T, Operand<T>, \
+                                         Operand<T>, int, int);                           \
+    template hipError_t dispatch_gels<T>(bool, int, int, int, Operand<T>, Operand<T>,     \
+                                         T *, int, int, int *);
+
+CBK_INSTANTIATE_DISPATCH(double)
+CBK_INSTANTIATE_DISPATCH(float)
+
+} /* namespace cbk */
