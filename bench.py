@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark: matrices solved per second by the compact batched SPD solve on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): dposv_compact,
+n = 32, nrhs = 4, 131072 SPD matrices A = Q diag(lambda) Q^T (Q from the QR of an
+N(0,1) matrix, log10(lambda) uniform on [0, k], k drawn per matrix from {2, 4, 6}),
+B ~ N(0,1), packed in the compact layout with V = 64, resident in HBM.
+
+A step is one call over the whole batch. Two paths are timed:
+  plain  dposv_compact   (FP64 factor + solve, A and B overwritten; each step
+                          runs on its own pre-filled copy, so no restore sits
+                          inside the timed region)
+  mixed  dsposv_compact  (FP32 factor, FP64 residual refinement, FP64 fallback;
+                          A and B are inputs, X the output), when the library
+                          exports it.
+value = the headline path's matrices/s counting only matrices whose backward
+error max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) is
+<= 1e-14 (computed in FP64 on the GPU after the timed steps).
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes per
+launch / its average duration from HIP events on the launch stream, against
+MI355X's 8 TB/s), and the reference's own CPU path (oracle/_ref, the
+reference's src/cbk.cpp compiled from its sources) timed on this host.
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, NRHS, NM, V = 32, 4, 131072, 64
+HBM_PEAK_GBS = 8000.0
+ETA_MAX = 1e-14
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nm", type=int, default=NM)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(torch, nm, seed=2026):
+    """dense FP64 A (nm, n, n) and B (nm, n, nrhs) on the GPU"""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A = torch.empty(nm, N, N, dtype=torch.float64, device="cuda")
+    chunk = 16384
+    for s in range(0, nm, chunk):
+        e = min(nm, s + chunk)
+        G = torch.randn(e - s, N, N, dtype=torch.float64, device="cuda", generator=g)
+        Q, Rr = torch.linalg.qr(G)
+        Q = Q * torch.sign(torch.diagonal(Rr, dim1=1, dim2=2)).unsqueeze(1)
+        k = torch.tensor([2.0, 4.0, 6.0], dtype=torch.float64, device="cuda")[
+            torch.randint(0, 3, (e - s,), device="cuda", generator=g)]
+        u = torch.rand(e - s, N, dtype=torch.float64, device="cuda", generator=g)
+        lam = torch.pow(10.0, u * k[:, None])
+        M = torch.matmul(Q * lam.unsqueeze(1), Q.transpose(1, 2).contiguous())
+        A[s:e] = 0.5 * (M + M.transpose(1, 2))
+    B = torch.randn(nm, N, NRHS, dtype=torch.float64, device="cuda", generator=g)
+    return A, B
+
+
+def pack(torch, X, V):
+    """dense (nm, rows, cols) -> compact column-major, ld = rows: (ng, cols, rows, V)"""
+    nm, r, c = X.shape
+    ng = (nm + V - 1) // V
+    full = torch.zeros(ng * V, r, c, dtype=X.dtype, device=X.device)
+    full[:nm] = X
+    for d in range(min(r, c)):
+        full[nm:, d, d] = 1.0
+    return full.view(ng, V, r, c).permute(0, 3, 2, 1).contiguous()
+
+
+def unpack(torch, P, nm):
+    ng, c, r, V = P.shape
+    return P.permute(0, 3, 2, 1).reshape(ng * V, r, c)[:nm]
+
+
+def backward_error(torch, A, X, B):
+    """per matrix max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf), FP64"""
+    R = B - torch.matmul(A, X)
+    an = A.abs().sum(dim=2).amax(dim=1)
+    num = R.abs().amax(dim=1)
+    den = an[:, None] * X.abs().amax(dim=1) + B.abs().amax(dim=1)
+    return (num / den).amax(dim=1)
+
+
+def cpu_baseline(nm):
+    """the reference's dposv_compact (oracle/_ref) on this host's cores, FP64,
+    n=32, nrhs=4, V=8, the same synthetic distribution; best of 3 passes with an
+    untimed restore (the reference's best_time, examples/bench_util.hpp:250-267)"""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cbkt
+    path = cbkt.ref_path()
+    kind = "reference"
+    if path is None:
+        lib, prefix, kind = ctypes.CDLL(cbkt.ORACLE_SO), "oracle_", "port"
+    else:
+        lib, prefix = ctypes.CDLL(path), ""
+    fn = getattr(lib, prefix + "dposv_compact")
+    fn.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                   ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    fn.restype = ctypes.c_int
+    r = np.random.default_rng(7)
+    G = r.standard_normal((nm, N, N))
+    Q, Rr = np.linalg.qr(G)
+    Q = Q * np.sign(np.diagonal(Rr, axis1=1, axis2=2))[:, None, :]
+    k = np.array([2.0, 4.0, 6.0])[r.integers(0, 3, nm)]
+    lam = 10.0 ** (r.uniform(0, 1, (nm, N)) * k[:, None])
+    A = (Q * lam[:, None, :]) @ np.swapaxes(Q, 1, 2)
+    A = 0.5 * (A + np.swapaxes(A, 1, 2))
+    B = r.standard_normal((nm, N, NRHS))
+    Vc = 8
+    ap0 = cbkt.pack(A, Vc, fill=0.0)
+    bp0 = cbkt.pack(B, Vc, pad_diag=0.0, fill=0.0)
+    ap, bp = ap0.copy(), bp0.copy()
+    best = float("inf")
+    for rep in range(4):  # one warm-up, then best of 3
+        ap[...] = ap0
+        bp[...] = bp0
+        t0 = time.perf_counter()
+        st = fn(b"C", b"L", N, NRHS, ap.ctypes.data, N, bp.ctypes.data, N, Vc, nm)
+        dt = time.perf_counter() - t0
+        assert st == 0
+        if rep > 0:
+            best = min(best, dt)
+    X = cbkt.unpack(bp, nm, N, NRHS, Vc)
+    Rs = B - A @ X
+    eta = np.max(np.abs(Rs).max(axis=1) / (np.abs(A).sum(axis=2).max(axis=1)[:, None] *
+                                            np.abs(X).max(axis=1) + np.abs(B).max(axis=1)))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": nm / best, "unit": "matrices/s", "cores": cores, "kind": kind,
+            "sample": f"dposv_compact n=32 nrhs=4 FP64 V=8, {nm} matrices, best of 3, "
+                      f"{os.path.basename(path) if path else 'oracle port'}; max backward error {eta:.1e}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}). "
+              "Matrices are independent, so N GPUs would each run this kernel on nm/N matrices.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import cbk_amd
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    cbk_amd.set_stream(stream)
+    nm = args.nm
+    ng = (nm + V - 1) // V
+    K, W = args.steps, args.warmup
+
+    log(f"generating {nm} SPD matrices n={N} (cond up to 1e6) ...")
+    A, B = make_inputs(torch, nm)
+    Ap, Bp = pack(torch, A, V), pack(torch, B, V)
+
+    # ---- plain FP64 dposv: one pre-filled copy per step
+    nsets = K + W
+    Aw = [Ap.clone() for _ in range(nsets)]
+    Bw = [Bp.clone() for _ in range(nsets)]
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(nsets)]
+    for s in range(W):
+        assert cbk_amd.call("d", "posv", "C", "L", N, NRHS, Aw[s], N, Bw[s], N, V, nm) == 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(W, nsets):
+        ev[s][0].record(stream)
+        assert cbk_amd.call("d", "posv", "C", "L", N, NRHS, Aw[s], N, Bw[s], N, V, nm) == 0
+        ev[s][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = [ev[s][0].elapsed_time(ev[s][1]) / 1e3 for s in range(W, nsets)]
+    avg_plain = sum(kt) / len(kt)
+    X = unpack(torch, Bw[-1], nm)
+    eta = backward_error(torch, A, X, B)
+    ok_plain = int((eta <= ETA_MAX).sum().item())
+    log(f"plain dposv: avg kernel {avg_plain * 1e6:.1f} us, wall/step {wall / K * 1e6:.1f} us, "
+        f"max eta {eta.max().item():.2e}, ok {ok_plain}/{nm}")
+    del Aw, Bw
+
+    bytes_per_matrix = 8 * (N * (N + 1) + 2 * N * NRHS)
+    plain_rate = ok_plain / (wall / K)
+    result = {
+        "metric": "matrices solved/sec (dposv, dgels n=32) mixed vs FP64 at backward err<=1e-14",
+        "value": plain_rate,
+        "unit": "matrices/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: A = Q diag(10^(k*U[0,1])) Q^T, k in {2,4,6} per matrix; B ~ N(0,1); seeded",
+        "config": {"workload": "dposv_compact n=32 nrhs=4, 131072 matrices, compact V=64, FP64",
+                   "nm": nm, "n": N, "nrhs": NRHS, "V": V, "path": "plain FP64"},
+        "plain_fp64": {"matrices_per_s": plain_rate, "kernel_us": avg_plain * 1e6,
+                       "ok_matrices": ok_plain, "max_backward_error": eta.max().item()},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": bytes_per_matrix * nm / avg_plain / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": bytes_per_matrix * nm / avg_plain / 1e9 / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "dposv_compact (sym_team_kernel<double,32,4,false>)",
+            "bytes_per_matrix": bytes_per_matrix,
+        },
+    }
+    if not args.no_cpu:
+        log("CPU baseline (reference dposv_compact on the host) ...")
+        result["cpu_baseline"] = cpu_baseline(nm)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -30,6 +30,14 @@ def lib():
             raise LibraryMissing(
                 f"{LIB_PATH} is not built; run `python -m cbk_amd.build` "
                 "(or __graft_entry__.build())")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7,
+        # and loading this library first would pull /opt/rocm's copy in beside
+        # it (two runtimes, two device views). With torch loaded first the
+        # dynamic linker resolves this library's libamdhip64.so.7 to torch's.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.cbk_set_stream.argtypes = [ctypes.c_void_p]
         _lib.cbk_set_stream.restype = ctypes.c_int
@@ -37,6 +45,10 @@ def lib():
         _lib.cbk_get_stream.restype = ctypes.c_void_p
         _lib.cbk_synchronize.argtypes = []
         _lib.cbk_synchronize.restype = ctypes.c_int
+        _lib.cbk_set_policy.argtypes = [ctypes.c_int]
+        _lib.cbk_set_policy.restype = ctypes.c_int
+        _lib.cbk_last_error_string.argtypes = []
+        _lib.cbk_last_error_string.restype = ctypes.c_char_p
         _lib.cbk_get_version.argtypes = []
         _lib.cbk_get_version.restype = ctypes.c_int
     return _lib
@@ -92,8 +104,23 @@ def set_stream(stream):
     return lib().cbk_set_stream(h)
 
 
+POLICY_AUTO = 0
+POLICY_LANE = 1
+
+
+def set_policy(policy):
+    """POLICY_AUTO (tuned kernels where they apply) or POLICY_LANE (the general
+    lane kernels everywhere); returns the previous policy."""
+    return lib().cbk_set_policy(policy)
+
+
 def synchronize():
     return lib().cbk_synchronize()
+
+
+def last_error():
+    """the HIP runtime's message for this thread's last CBK_ERR_LAUNCH"""
+    return lib().cbk_last_error_string().decode()
 
 
 def version():

@@ -18,9 +18,12 @@ namespace cbk {
 
 namespace {
 thread_local hipStream_t t_stream = nullptr;
+thread_local int t_policy = CBK_POLICY_AUTO;
+thread_local hipError_t t_last_error = hipSuccess;
 }
 
 hipStream_t current_stream() { return t_stream; }
+int current_policy() { return t_policy; }
 
 namespace {
 
@@ -32,7 +35,12 @@ inline bool vlen_ok(int V)
     return V == 2 || V == 4 || V == 8 || V == 16 || V == 32 || V == 64;
 }
 
-inline int status(hipError_t e) { return e == hipSuccess ? 0 : CBK_ERR_LAUNCH; }
+inline int status(hipError_t e)
+{
+    if (e == hipSuccess) return 0;
+    t_last_error = e;
+    return CBK_ERR_LAUNCH;
+}
 
 /* group_stride (src/cbk_common.hpp:210-214) */
 inline size_t gstride(bool row, int ld, int rows, int cols, int V)
@@ -258,6 +266,15 @@ int cbk_set_stream(void *stream)
 void *cbk_get_stream(void) { return static_cast<void *>(t_stream); }
 
 int cbk_synchronize(void) { return status(hipStreamSynchronize(t_stream)); }
+
+const char *cbk_last_error_string(void) { return hipGetErrorString(t_last_error); }
+
+int cbk_set_policy(int policy)
+{
+    const int prev = t_policy;
+    if (policy == CBK_POLICY_AUTO || policy == CBK_POLICY_LANE) t_policy = policy;
+    return prev;
+}
 
 #define CBK_ENTRY_POINTS(T, p)                                                           \
     int p##geqrf_compact(char layout, int m, int n, T *ap, int ldap, T *taup, int V,     \

@@ -12,6 +12,9 @@ namespace cbk {
 /* the stream of the calling host thread (cbk_set_stream) */
 hipStream_t current_stream();
 
+/* the kernel policy of the calling host thread (cbk_set_policy) */
+int current_policy();
+
 /* One operand of a compact batch: base pointer, leading dimension, layout and
  * scalars per group (group_stride of the reference, src/cbk_common.hpp:210). */
 template <class T> struct Operand {
@@ -49,6 +52,16 @@ hipError_t launch_orgqr_lane(int m, int n, int k, Operand<T> A, const T *tau, in
 template <class T>
 hipError_t launch_gels_lane(bool tran, int m, int n, int nrhs, Operand<T> A, Operand<T> B,
                             T *tau, int V, int nm, int *info);
+
+/* ---- team kernels: one matrix per 16-lane team, held in registers ---- */
+
+/* orders served by the potrf/posv/sytrfnp/sysvnp team kernel */
+bool sym_team_supports(int n);
+
+/* same contract as launch_chol_lane */
+template <class T>
+hipError_t launch_sym_team(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
+                           Operand<T> B, int V, int nm, int *info);
 
 /* B(0:rows, 0:cols) := 0 in every group */
 template <class T>

@@ -2,12 +2,16 @@
 
 #include "dispatch.h"
 
+#include "cbk_ext.h"
+
 namespace cbk {
 
 template <class T>
 hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
                         Operand<T> B, int V, int nm, int *info)
 {
+    if (current_policy() == CBK_POLICY_AUTO && sym_team_supports(n) && V >= 4)
+        return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
 }
 

@@ -1,0 +1,8 @@
+/* the potrf/posv/sytrfnp/sysvnp team kernels of order 32, double */
+#include "sym_team_impl.h"
+
+namespace cbk {
+namespace symteam {
+template hipError_t launch_n<double, 32>(bool, const SymArgs<double> &);
+}
+} /* namespace cbk */

@@ -1,0 +1,8 @@
+/* the potrf/posv/sytrfnp/sysvnp team kernels of order 16, float */
+#include "sym_team_impl.h"
+
+namespace cbk {
+namespace symteam {
+template hipError_t launch_n<float, 16>(bool, const SymArgs<float> &);
+}
+} /* namespace cbk */

@@ -1,0 +1,98 @@
+/* team_common.h -- helpers of the team kernels: one matrix per 16-lane team.
+ *
+ * A team is one DPP row of a wavefront (lanes 16k .. 16k+15), so a value held
+ * by lane s of the team reaches every lane of the team with a single
+ * v_mov_b32_dpp row_newbcast:s per 32-bit half -- no LDS round trip, no
+ * barrier. Indices that select a source lane or a register must be
+ * compile-time constants, hence the sfor<> loops below: every loop over rows,
+ * columns and pivots of a team kernel is unrolled at compile time.
+ */
+#ifndef CBK_TEAM_COMMON_H
+#define CBK_TEAM_COMMON_H
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+namespace cbk {
+namespace team {
+
+constexpr int TS = 16; /* lanes per team */
+
+template <int B, int E, class F> __device__ __forceinline__ void sfor(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        sfor<B + 1, E>(f);
+    }
+}
+
+/* descending: E-1, E-2, ..., B */
+template <int B, int E, class F> __device__ __forceinline__ void sfor_down(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, E - 1>{});
+        sfor_down<B, E - 1>(f);
+    }
+}
+
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+
+/* value of lane S of this lane's 16-lane row, in every lane of the row */
+template <int S> __device__ __forceinline__ float bcast(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x),
+                                                              0x150 + S, 0xF, 0xF, true));
+}
+template <int S> __device__ __forceinline__ double bcast(double x)
+{
+    int2 w = __builtin_bit_cast(int2, x);
+    w.x = __builtin_amdgcn_mov_dpp(w.x, 0x150 + S, 0xF, 0xF, true);
+    w.y = __builtin_amdgcn_mov_dpp(w.y, 0x150 + S, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, w);
+}
+
+/* ---- buffer addressing --------------------------------------------------
+ * A team kernel addresses one compact group through a buffer resource whose
+ * base is the group (wave-uniform: the 4 teams of a wave are 4 consecutive
+ * slots of one group, V being a multiple of 4), a 32-bit per-lane byte offset
+ * (row and slot) and a scalar byte offset (column): one VGPR per row block
+ * instead of a 64-bit address per element. */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t group_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, 0xffffffff,
+                                             0x00020000);
+}
+
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double bload(double, __amdgpu_buffer_rsrc_t r, unsigned voff,
+                                        unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ float bload(float, __amdgpu_buffer_rsrc_t r, unsigned voff,
+                                       unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore(double x, __amdgpu_buffer_rsrc_t r, unsigned voff,
+                                       unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore(float x, __amdgpu_buffer_rsrc_t r, unsigned voff,
+                                       unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, voff, soff, 0);
+}
+
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double sqrtT(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float sqrtT(float x) { return __builtin_sqrtf(x); }
+
+} /* namespace team */
+} /* namespace cbk */
+
+#endif /* CBK_TEAM_COMMON_H */

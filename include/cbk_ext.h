@@ -22,6 +22,19 @@ int cbk_set_stream(void *stream);
 void *cbk_get_stream(void);
 int cbk_synchronize(void);
 
+/* the HIP runtime's message for the last CBK_ERR_LAUNCH of this thread */
+const char *cbk_last_error_string(void);
+
+/* ---- kernel selection ---------------------------------------------------
+ * CBK_POLICY_AUTO (default) lets the dispatcher pick the fastest kernel for the
+ * shape (team kernels for the tuned orders, the lane kernel otherwise);
+ * CBK_POLICY_LANE forces the general one-lane-per-matrix kernels, whose
+ * results are bit-identical to the reference kernels for every routine.
+ * Thread-local, like the stream. Returns the previous policy. */
+#define CBK_POLICY_AUTO 0
+#define CBK_POLICY_LANE 1
+int cbk_set_policy(int policy);
+
 /* ---- per-matrix info ---------------------------------------------------
  * The plain routine's arguments plus `info`, a device int[nm]: 0, or the LAPACK
  * code of that matrix. The factors are bit-identical to the plain call.

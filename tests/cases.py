@@ -134,6 +134,16 @@ def same_bits(x, y):
     return np.array_equal(bits(x), bits(y))
 
 
+def same_bits_nan(x, y):
+    """bit equality with every NaN taken as one value (NaN payloads and signs
+    differ between x86 and the GPU; a poisoned lane is NaN on both)"""
+    x, y = np.asarray(x), np.asarray(y)
+    nx, ny = np.isnan(x), np.isnan(y)
+    if not np.array_equal(nx, ny):
+        return False
+    return np.array_equal(bits(np.where(nx, 0, x)), bits(np.where(ny, 0, y)))
+
+
 # ---------------------------------------------------------------- case sweeps
 
 SMALL = (0, 1, 2, 3, 5)

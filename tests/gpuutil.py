@@ -7,8 +7,16 @@ import numpy as np
 import cbk_amd
 
 
+_ready = False
+
+
 def torch():
+    global _ready
     import torch as _t
+    if not _ready:
+        _t.cuda.init()
+        _t.zeros(1, device="cuda")  # a live context before the first library call
+        _ready = True
     return _t
 
 
@@ -31,7 +39,7 @@ def gpu_call(p, routine, *args, info=None):
     if info is not None:
         dinfo = to_dev(np.full(info.shape, -99, dtype=np.int32))
     st = cbk_amd.call(p, routine, *targs, info=dinfo)
-    assert cbk_amd.synchronize() == 0
+    assert cbk_amd.synchronize() == 0, cbk_amd.last_error()
     for a, d in dev:
         a[...] = d.cpu().numpy()
     if info is not None:

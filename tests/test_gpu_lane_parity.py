@@ -22,6 +22,15 @@ def _gpu():
     return gpuutil
 
 
+@pytest.fixture(autouse=True)
+def lane_policy():
+    """these tests pin the general lane kernels: force them for every shape"""
+    import cbk_amd
+    prev = cbk_amd.set_policy(cbk_amd.POLICY_LANE)
+    yield
+    cbk_amd.set_policy(prev)
+
+
 def _check(case):
     so, outo = cases.run(cbkt.oracle().call, case)
     sg, outg = cases.run(_gpu().gpu_call, case)
