@@ -92,6 +92,27 @@ def call(scalar, routine, *args, info=None, variant=None):
     return f(*cargs)
 
 
+def call_ext(name, *args):
+    """Call an include/cbk_ext.h entry point listed in abi.EXT (device tensors
+    or addresses for its pointers)."""
+    key = ("ext", name, "")
+    if key not in _fns:
+        kinds = abi.EXT[name]
+        _fns[key] = (abi.bind(lib(), name, kinds, "d"), kinds)
+    f, kinds = _fns[key]
+    if len(args) != len(kinds):
+        raise TypeError(f"{name}: expected {len(kinds)} arguments, got {len(args)}")
+    return f(*[_ptr(a) if k == "p" else abi.to_c(k, a) for k, a in zip(kinds, args)])
+
+
+def dsposv_worksize(layout, uplo, n, nrhs, V, nm):
+    f = lib().dsposv_compact_worksize
+    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                  ctypes.c_int]
+    f.restype = ctypes.c_size_t
+    return f(abi.to_c("c", layout), abi.to_c("c", uplo), n, nrhs, V, nm)
+
+
 def set_stream(stream):
     """Make `stream` (a torch.cuda.Stream, a raw hipStream_t int, or None) the
     stream this thread's calls launch on."""

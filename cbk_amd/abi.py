@@ -23,6 +23,11 @@ ROUTINES = {
     "gels": "cciiipipipii",
 }
 
+# entry points of include/cbk_ext.h beyond the _info variants ('z' = size_t)
+EXT = {
+    "dsposv_compact": "cciipipipipzppii",
+}
+
 # per-matrix info variants: the plain argument list plus a trailing device int[nm]
 INFO_ROUTINES = ("potrf", "posv", "sytrfnp", "sysvnp", "gels")
 
@@ -34,6 +39,8 @@ def ctype_of(kind, scalar):
         return ctypes.c_int
     if kind == "p":
         return ctypes.c_void_p
+    if kind == "z":
+        return ctypes.c_size_t
     if kind == "T":
         return ctypes.c_double if scalar == "d" else ctypes.c_float
     raise ValueError(kind)

@@ -248,6 +248,50 @@ int gels(char layout, char trans, int m, int n, int nrhs, T *ap, int ldap, T *bp
     return status(dispatch_gels<T>(tran, m, n, nrhs, A, B, taup, V, nm, info));
 }
 
+constexpr int kItermax = 30; /* LAPACK dsposv's ITERMAX */
+
+bool mixed_uses_team(int n, int nrhs, int V)
+{
+    return current_policy() == CBK_POLICY_AUTO && V >= 4 && mixed_team_supports(n, nrhs);
+}
+
+size_t mixed_worksize(int n, int nrhs, int V, int nm)
+{
+    if (mixed_uses_team(n, nrhs, V)) return 0;
+    const size_t groups = (size_t)((nm + V - 1) / V);
+    return groups * V * ((size_t)n * n + (size_t)n * nrhs) * sizeof(float);
+}
+
+int dsposv(char layout, char uplo, int n, int nrhs, double *ap, int ldap, const double *bp,
+           int ldbp, double *xp, int ldxp, void *work, size_t lwork, int *iter, int *info,
+           int V, int nm)
+{
+    const bool col = opt(layout, 'C'), row = opt(layout, 'R');
+    const bool lo = opt(uplo, 'L'), up = opt(uplo, 'U');
+    if (!col && !row) return -1;
+    if (!lo && !up) return -2;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (ldap < max1(n)) return -6;
+    if (ldbp < max1(row ? nrhs : n)) return -8;
+    if (ldxp < max1(row ? nrhs : n)) return -10;
+    if (vlen_ok(V) && nm >= 0 && lwork < mixed_worksize(n, nrhs, V, nm)) return -12;
+    if (!vlen_ok(V)) return -15;
+    if (nm < 0) return -16;
+    if (n == 0 || nrhs == 0 || nm == 0) {
+        hipError_t e = launch_clear_info(iter, nm);
+        if (e == hipSuccess) e = launch_clear_info(info, nm);
+        return status(e);
+    }
+    const Operand<double> A = operand(ap, ldap, row, n, n, V);
+    const Operand<double> B = operand(const_cast<double *>(bp), ldbp, row, n, nrhs, V);
+    const Operand<double> X = operand(xp, ldxp, row, n, nrhs, V);
+    if (mixed_uses_team(n, nrhs, V))
+        return status(launch_mixed_posv_team(up, n, nrhs, A, B, X, V, nm, kItermax, iter, info));
+    return status(launch_mixed_posv_lane(up, n, nrhs, A, B, X, static_cast<float *>(work), V,
+                                         nm, kItermax, iter, info));
+}
+
 } /* namespace */
 } /* namespace cbk */
 
@@ -364,6 +408,22 @@ int cbk_set_policy(int policy)
         return gels<T>(layout, trans, m, n, nrhs, ap, ldap, bp, ldbp, taup, V, nm,       \
                        info);                                                            \
     }
+
+int dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
+                   const double *bp, int ldbp, double *xp, int ldxp, void *work,
+                   size_t lwork, int *iter, int *info, int V, int nm)
+{
+    return dsposv(layout, uplo, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, work, lwork, iter,
+                  info, V, nm);
+}
+
+size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int V, int nm)
+{
+    (void)layout;
+    (void)uplo;
+    if (n < 0 || nrhs < 0 || nm < 0 || !vlen_ok(V)) return 0;
+    return mixed_worksize(n, nrhs, V, nm);
+}
 
 CBK_ENTRY_POINTS(double, d)
 CBK_ENTRY_POINTS(float, s)

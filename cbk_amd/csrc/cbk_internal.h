@@ -53,6 +53,11 @@ template <class T>
 hipError_t launch_gels_lane(bool tran, int m, int n, int nrhs, Operand<T> A, Operand<T> B,
                             T *tau, int V, int nm, int *info);
 
+/* mixed-precision posv, lane path: work holds groups*V*(n*n + n*nrhs) floats */
+hipError_t launch_mixed_posv_lane(bool up, int n, int nrhs, Operand<double> A,
+                                  Operand<double> B, Operand<double> X, float *work, int V,
+                                  int nm, int itermax, int *iter, int *info);
+
 /* ---- team kernels: one matrix per 16-lane team, held in registers ---- */
 
 /* orders served by the potrf/posv/sytrfnp/sysvnp team kernel */
@@ -62,6 +67,14 @@ bool sym_team_supports(int n);
 template <class T>
 hipError_t launch_sym_team(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
                            Operand<T> B, int V, int nm, int *info);
+
+/* orders and right-hand-side counts served by the fused mixed posv team kernel */
+bool mixed_team_supports(int n, int nrhs);
+
+/* mixed posv, team path (no workspace); A is the operand as passed (uplo via up) */
+hipError_t launch_mixed_posv_team(bool up, int n, int nrhs, Operand<double> A,
+                                  Operand<double> B, Operand<double> X, int V, int nm,
+                                  int itermax, int *iter, int *info);
 
 /* B(0:rows, 0:cols) := 0 in every group */
 template <class T>

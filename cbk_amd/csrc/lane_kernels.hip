@@ -433,6 +433,120 @@ __global__ __launch_bounds__(kBlock) void gels_lane_kernel(bool tran, int m, int
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* mixed precision posv (LAPACK dsposv), one lane per matrix          */
+/* ------------------------------------------------------------------ */
+
+/* The general path of dsposv_compact: the FP32 factor and the FP32 right-hand
+ * side / correction live in the caller's workspace (compact, column-major,
+ * ld = n, the lane's slot), A and B are read through their views, X is
+ * written. Same algorithm and codes as the team kernel (mixed_team_impl.h):
+ * refinement until ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n) for every
+ * column, FP64 fallback in place otherwise. */
+__global__ __launch_bounds__(kBlock) void mixed_posv_lane_kernel(
+    bool up, int n, int nrhs, Operand<double> A, Operand<double> B, Operand<double> X,
+    float *work, int V, long nlanes, int nm, int itermax, int *iter_out, int *info_out)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nlanes) return;
+    const long g = idx / V;
+    const int v = (int)(idx % V);
+    const LaneView<double> Av0 = lane_view(A, g, v, V);
+    const LaneView<double> Al = up ? Av0.tr() : Av0; /* lower view */
+    const LaneView<double> Bv = lane_view(B, g, v, V), Xv = lane_view(X, g, v, V);
+    float *wg = work + (size_t)g * V * (size_t)(n * n + n * nrhs) + v;
+    const LaneView<float> S{wg, 1, n, V};                         /* FP32 factor */
+    const LaneView<float> W{wg + (size_t)n * n * V, 1, n, V};     /* FP32 rhs */
+    auto sym = [&](int i, int c) { return i >= c ? Al(i, c) : Al(c, i); };
+    constexpr double kFltMax = 3.4028234663852886e38;
+
+    double anrm = 0.0;
+    bool over = false;
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int c = 0; c < n; ++c) {
+            const double x = __builtin_fabs(sym(i, c));
+            s += x;
+            over |= x > kFltMax;
+        }
+        anrm = fmax(anrm, s);
+        for (int q = 0; q < nrhs; ++q) over |= __builtin_fabs(Bv(i, q)) > kFltMax;
+    }
+    const double cte = anrm * 0x1p-53 * __builtin_sqrt((double)n);
+    int it_code = 0;
+    bool done = false;
+    if (over) {
+        it_code = -2;
+    }
+    else {
+        for (int c = 0; c < n; ++c)
+            for (int i = c; i < n; ++i)
+                S(i, c) = (float)Al(i, c);
+        int finfo = 0;
+        chol_lane<float, false>(n, S, &finfo);
+        if (finfo != 0) {
+            it_code = -3;
+        }
+        else {
+            for (int q = 0; q < nrhs; ++q)
+                for (int i = 0; i < n; ++i)
+                    W(i, q) = (float)Bv(i, q);
+            chol_solve_lane<float, false>(false, n, nrhs, S, W);
+            for (int q = 0; q < nrhs; ++q)
+                for (int i = 0; i < n; ++i)
+                    Xv(i, q) = (double)W(i, q);
+            for (int it = 0;; ++it) {
+                bool conv = true, rover = false;
+                for (int q = 0; q < nrhs; ++q) {
+                    double rn = 0.0, xn = 0.0;
+                    for (int i = 0; i < n; ++i) {
+                        double r = Bv(i, q);
+                        for (int c = 0; c < n; ++c)
+                            r = fmaT(-sym(i, c), Xv(c, q), r);
+                        const double ar = __builtin_fabs(r), ax = __builtin_fabs(Xv(i, q));
+                        rn = (ar > rn || ar != ar) ? ar : rn;
+                        xn = (ax > xn || ax != ax) ? ax : xn;
+                        rover |= ar > kFltMax;
+                        W(i, q) = (float)r;
+                    }
+                    if (!(rn <= xn * cte)) conv = false;
+                }
+                if (conv) {
+                    it_code = it;
+                    done = true;
+                    break;
+                }
+                if (it == itermax) {
+                    it_code = -itermax - 1;
+                    break;
+                }
+                if (rover) {
+                    it_code = -2;
+                    break;
+                }
+                chol_solve_lane<float, false>(false, n, nrhs, S, W);
+                for (int q = 0; q < nrhs; ++q)
+                    for (int i = 0; i < n; ++i)
+                        Xv(i, q) = Xv(i, q) + (double)W(i, q);
+            }
+        }
+    }
+    int dinfo = 0;
+    if (!done) {
+        chol_lane<double, false>(n, Al, &dinfo);
+        if (dinfo == 0) {
+            for (int q = 0; q < nrhs; ++q)
+                for (int i = 0; i < n; ++i)
+                    Xv(i, q) = Bv(i, q);
+            chol_solve_lane<double, false>(up, n, nrhs, Av0, Xv);
+        }
+    }
+    if (idx < nm) {
+        iter_out[idx] = it_code;
+        info_out[idx] = dinfo;
+    }
+}
+
 inline long nlanes_of(int V, int nm) { return (long)((nm + V - 1) / V) * V; }
 inline unsigned nblocks(long nlanes) { return (unsigned)((nlanes + kBlock - 1) / kBlock); }
 
@@ -513,6 +627,17 @@ hipError_t launch_gels_lane(bool tran, int m, int n, int nrhs, Operand<T> A, Ope
     const long nl = nlanes_of(V, nm);
     hipLaunchKernelGGL((gels_lane_kernel<T>), dim3(nblocks(nl)), dim3(kBlock), 0,
                        current_stream(), tran, m, n, nrhs, A, B, tau, V, nl, nm, info);
+    return hipGetLastError();
+}
+
+hipError_t launch_mixed_posv_lane(bool up, int n, int nrhs, Operand<double> A,
+                                  Operand<double> B, Operand<double> X, float *work, int V,
+                                  int nm, int itermax, int *iter, int *info)
+{
+    const long nl = nlanes_of(V, nm);
+    hipLaunchKernelGGL(mixed_posv_lane_kernel, dim3(nblocks(nl)), dim3(kBlock), 0,
+                       current_stream(), up, n, nrhs, A, B, X, work, V, nl, nm, itermax, iter,
+                       info);
     return hipGetLastError();
 }
 

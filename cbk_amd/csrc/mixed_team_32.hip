@@ -1,0 +1,8 @@
+/* the fused mixed-precision posv team kernel of order 32 */
+#include "mixed_team_impl.h"
+
+namespace cbk {
+namespace mixedteam {
+template hipError_t launch_n<32>(const MixedArgs &);
+}
+} /* namespace cbk */

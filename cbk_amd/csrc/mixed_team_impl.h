@@ -1,0 +1,246 @@
+/* mixed_team_impl.h -- mixed-precision SPD solve (LAPACK dsposv semantics),
+ * one matrix per 16-lane team, fused in one pass over HBM.
+ *
+ * Per matrix: A (FP64) and B are read once; A stays in registers as full
+ * symmetric rows (the upper half read from the mirrored lower storage), the
+ * FP32 Cholesky factor and its LDS copy are built from it, X0 = the FP32
+ * solve, then LAPACK dsposv's refinement: r = b - A x in FP64 (the only FP64
+ * arithmetic), d = the FP32 solve of r, x += d in FP64, until every column
+ * satisfies ||r||_inf <= ||x||_inf * ||A||_inf * eps * sqrt(n) (eps = 2^-53,
+ * dlamch('E')) or ITERMAX steps pass. On success A is untouched and only X is
+ * written: 8 n(n+1)/2 + 16 n nrhs bytes of compulsory traffic against the plain
+ * dposv's 8 n(n+1) + 16 n nrhs. If the FP32 factor fails, a value overflows
+ * FP32, or refinement does not converge, the matrix falls back to the FP64
+ * Cholesky (team_chol.h) in place, A receives that factor, and iter records
+ * why (-2 overflow, -3 FP32 factor failed, -(ITERMAX+1) no convergence), as
+ * dsposv's ITER does.
+ *
+ * The FP32 factor and solves are the team building blocks of team_chol.h; the
+ * symmetric residual sums A(i,c) x_c over c ascending with one FP64 fma each.
+ */
+#ifndef CBK_MIXED_TEAM_IMPL_H
+#define CBK_MIXED_TEAM_IMPL_H
+
+#include "cbk_internal.h"
+#include "team_chol.h"
+
+namespace cbk {
+namespace mixedteam {
+
+using namespace team;
+
+struct MixedArgs {
+    double *a; /* lower view */
+    int a_si, a_sj;
+    size_t a_gs;
+    const double *b;
+    int b_si, b_sj;
+    size_t b_gs;
+    double *x;
+    int x_si, x_sj;
+    size_t x_gs;
+    int V;
+    long nmat;
+    int nm;
+    int nrhs;
+    int itermax;
+    int *iter;
+    int *info;
+};
+
+constexpr int kTeamsPerBlock = 4;
+
+template <int N> hipError_t launch_n(const MixedArgs &args);
+
+template <int N, int NRB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void
+mixed_posv_kernel(MixedArgs p)
+{
+    constexpr int R = rows_per_lane<N>();
+    /* one team region, used as FP32 L during refinement, FP64 L on fallback */
+    __shared__ double lds[kTeamsPerBlock * chol_lds<N>()];
+
+    const int t = threadIdx.x & (TS - 1);
+    const int team = threadIdx.x / TS;
+    const long mat0 = (long)blockIdx.x * kTeamsPerBlock;
+    const long mat = mat0 + team;
+    const long g = mat0 / p.V;
+    const long groups_left = p.nmat / p.V - g;
+    const unsigned v = (unsigned)(mat0 % p.V) + team;
+    const unsigned V = (unsigned)p.V;
+    constexpr unsigned SZ = 8;
+    const __amdgpu_buffer_rsrc_t ra = group_rsrc(p.a + g * p.a_gs, group_range(groups_left, p.a_gs, SZ));
+    const __amdgpu_buffer_rsrc_t rb = group_rsrc(p.b + g * p.b_gs, group_range(groups_left, p.b_gs, SZ));
+    const __amdgpu_buffer_rsrc_t rx = group_rsrc(p.x + g * p.x_gs, group_range(groups_left, p.x_gs, SZ));
+    const unsigned a_col = (unsigned)p.a_sj * V * SZ, a_rowb = (unsigned)p.a_si * V * SZ;
+    unsigned a_row[R], a_cbase[R], b_row[R], x_row[R];
+    sfor<0, R>([&](auto r) {
+        const int i = t + TS * r;
+        const bool in = i < N;
+        a_row[r] = in ? ((unsigned)(i * p.a_si) * V + v) * SZ : kOOB;   /* (i, c): + c*a_col */
+        a_cbase[r] = in ? ((unsigned)(i * p.a_sj) * V + v) * SZ : kOOB; /* (c, i): + c*a_rowb */
+        b_row[r] = in ? ((unsigned)(i * p.b_si) * V + v) * SZ : kOOB;
+        x_row[r] = in ? ((unsigned)(i * p.x_si) * V + v) * SZ : kOOB;
+    });
+    unsigned bcol[NRB], xcol[NRB];
+    sfor<0, NRB>([&](auto q) {
+        bcol[q] = (q < p.nrhs) ? q * (unsigned)p.b_sj * V * SZ : kOOB;
+        xcol[q] = (q < p.nrhs) ? q * (unsigned)p.x_sj * V * SZ : kOOB;
+    });
+
+    /* full symmetric rows of A in FP64: (i, c) from the lower view's (i, c)
+     * when c <= i, from its (c, i) otherwise */
+    double a64[R][N];
+    sfor<0, R>([&](auto r) {
+        const int i = t + TS * r;
+        sfor<0, N>([&](auto c) {
+            const unsigned off = (a_row[r] == kOOB) ? kOOB
+                                 : (c <= i)         ? a_row[r] + c * a_col
+                                                    : a_cbase[r] + c * a_rowb;
+            a64[r][c] = bload(0.0, ra, off, 0);
+        });
+    });
+    double b64[R][NRB];
+    sfor<0, R>([&](auto r) {
+        sfor<0, NRB>([&](auto q) { b64[r][q] = bload(0.0, rb, b_row[r], bcol[q]); });
+    });
+
+    /* ||A||_inf (dlansy 'I') and the FP32 overflow test (dlat2s / dlag2s) */
+    double rowsum[R];
+    bool over = false;
+    sfor<0, R>([&](auto r) {
+        rowsum[r] = 0.0;
+        sfor<0, N>([&](auto c) {
+            rowsum[r] += __builtin_fabs(a64[r][c]);
+            over |= __builtin_fabs(a64[r][c]) > 3.4028234663852886e38;
+        });
+        sfor<0, NRB>([&](auto q) { over |= __builtin_fabs(b64[r][q]) > 3.4028234663852886e38; });
+    });
+    double anrm = 0.0;
+    sfor<0, R>([&](auto r) {
+        if (t + TS * r < N) anrm = fmax_nan(anrm, rowsum[r]);
+    });
+    anrm = team_max(anrm);
+    const double cte = anrm * 0x1p-53 * __builtin_sqrt((double)N);
+    /* any lane of the team overflowing sends the whole matrix to FP64 */
+    const bool team_over = team_max(over ? 1.0 : 0.0) > 0.0;
+
+    int iter = 0;
+    bool done = false;
+    float *L32 = reinterpret_cast<float *>(lds + team * chol_lds<N>());
+    double x64[R][NRB];
+    if (!team_over) {
+        float a32[R][N], dg32[R], dinv32[R];
+        sfor<0, R>([&](auto r) {
+            sfor<0, cmin(TS * (r + 1), N)>([&](auto c) { a32[r][c] = (float)a64[r][c]; });
+        });
+        const int finfo = chol_factor<float, N, false>(a32, dg32, t);
+        if (finfo != 0) {
+            iter = -3;
+        }
+        else {
+            chol_prepare<float, N, false>(a32, dg32, dinv32, L32, t);
+            float w[R][NRB];
+            sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { w[r][q] = (float)b64[r][q]; }); });
+            chol_solve<float, N, NRB, false>(a32, dinv32, w, L32, t);
+            sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { x64[r][q] = (double)w[r][q]; }); });
+            for (int it = 0;; ++it) {
+                /* r = b - A x, FP64 */
+                double res[R][NRB];
+                sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { res[r][q] = b64[r][q]; }); });
+                sfor<0, N>([&](auto c) {
+                    sfor<0, NRB>([&](auto q) {
+                        const double xc = bcast<c % TS>(x64[c / TS][q]);
+                        sfor<0, R>([&](auto r) { res[r][q] = fmaT(-a64[r][c], xc, res[r][q]); });
+                    });
+                });
+                /* every column: ||r||_inf <= ||x||_inf * cte (NaN fails) */
+                bool conv = true;
+                sfor<0, NRB>([&](auto q) {
+                    double rn = 0.0, xn = 0.0;
+                    sfor<0, R>([&](auto r) {
+                        if (t + TS * r < N) {
+                            rn = fmax_nan(rn, __builtin_fabs(res[r][q]));
+                            xn = fmax_nan(xn, __builtin_fabs(x64[r][q]));
+                        }
+                    });
+                    rn = team_max(rn);
+                    xn = team_max(xn);
+                    if (q < p.nrhs && !(rn <= xn * cte)) conv = false;
+                });
+                if (conv) {
+                    iter = it;
+                    done = true;
+                    break;
+                }
+                if (it == p.itermax) {
+                    iter = -p.itermax - 1;
+                    break;
+                }
+                bool rover = false;
+                sfor<0, R>([&](auto r) {
+                    sfor<0, NRB>([&](auto q) {
+                        rover |= __builtin_fabs(res[r][q]) > 3.4028234663852886e38;
+                        w[r][q] = (float)res[r][q];
+                    });
+                });
+                if (team_max(rover ? 1.0 : 0.0) > 0.0) {
+                    iter = -2;
+                    break;
+                }
+                chol_solve<float, N, NRB, false>(a32, dinv32, w, L32, t);
+                sfor<0, R>([&](auto r) {
+                    sfor<0, NRB>([&](auto q) { x64[r][q] = x64[r][q] + (double)w[r][q]; });
+                });
+            }
+        }
+    }
+    else {
+        iter = -2;
+    }
+
+    int dinfo = 0;
+    if (!done) {
+        /* FP64 fallback: dposv on A in place; A receives the factor */
+        double a[R][N], dg[R], dinv[R];
+        sfor<0, R>([&](auto r) {
+            sfor<0, cmin(TS * (r + 1), N)>([&](auto c) { a[r][c] = a64[r][c]; });
+        });
+        dinfo = chol_factor<double, N, false>(a, dg, t);
+        sfor<0, R>([&](auto r) {
+            const int i = t + TS * r;
+            sfor<0, cmin(TS * (r + 1), N)>([&](auto c) {
+                const unsigned off = (c < TS * r || c < i) ? a_row[r] : kOOB;
+                bstore(a[r][c], ra, off, c * a_col);
+            });
+            const unsigned doff = (a_row[r] == kOOB) ? kOOB : a_row[r] + (unsigned)i * a_col;
+            bstore(dg[r], ra, doff, 0);
+        });
+        if (dinfo == 0) {
+            chol_prepare<double, N, false>(a, dg, dinv, lds + team * chol_lds<N>(), t);
+            sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { x64[r][q] = b64[r][q]; }); });
+            chol_solve<double, N, NRB, false>(a, dinv, x64, lds + team * chol_lds<N>(), t);
+        }
+    }
+    if (mat < p.nm && t == 0) {
+        p.iter[mat] = iter;
+        p.info[mat] = dinfo;
+    }
+    if (dinfo == 0) {
+        sfor<0, R>([&](auto r) {
+            sfor<0, NRB>([&](auto q) { bstore(x64[r][q], rx, x_row[r], xcol[q]); });
+        });
+    }
+}
+
+template <int N> hipError_t launch_n(const MixedArgs &args)
+{
+    const unsigned blocks = (unsigned)((args.nmat + kTeamsPerBlock - 1) / kTeamsPerBlock);
+    hipLaunchKernelGGL((mixed_posv_kernel<N, 4>), dim3(blocks), dim3(64), 0, current_stream(), args);
+    return hipGetLastError();
+}
+
+} /* namespace mixedteam */
+} /* namespace cbk */
+
+#endif /* CBK_MIXED_TEAM_IMPL_H */

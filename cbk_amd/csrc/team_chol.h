@@ -1,0 +1,200 @@
+/* team_chol.h -- the register-resident Cholesky / LDL^T building blocks of
+ * the team kernels (sym_team_impl.h, mixed_team_impl.h).
+ *
+ * A team is 16 lanes (one DPP row) holding one n x n matrix: lane t owns rows
+ * t, t+16, ... of the lower view, a[r][c] = element (t+16r, c), c < 16(r+1).
+ *
+ * chol_factor: the reference's factorization exactly
+ * (src/cbk_potrf_compact.hpp:79-124, src/cbk_sytrfnp_compact.hpp:96-265) in
+ * right-looking per-pivot order (lane_kernels.hip explains why the operations
+ * per element are the reference's, in its order): pivot j is broadcast from
+ * its owner lane; columns of j's 8-wide panel take the panel form on the
+ * unscaled column with weight 1/a, later columns the trailing form on the
+ * scaled column. Entries above the diagonal ride along as scratch: they only
+ * ever feed other scratch entries and are never stored.
+ *
+ * chol_prepare / chol_solve: forward substitution with L (right-looking; each
+ * row accumulates in the reference's order), then backward substitution with
+ * L^T, column-oriented over a copy of L in LDS so a lane reads L(l, i) for its
+ * own rows i. For potrs the diagonal is folded into the factor once (columns
+ * of L scaled by 1/L(i,i) for the forward sweep, rows for the backward one),
+ * so a sweep step is one broadcast and one fma per row block; sytrsnp keeps
+ * the reference's unit sweeps and reciprocal D scaling
+ * (src/cbk_sytrsnp_compact.hpp:52-78). The backward sweep sums in descending
+ * l where the reference's row-dot ascends, so X matches to rounding.
+ */
+#ifndef CBK_TEAM_CHOL_H
+#define CBK_TEAM_CHOL_H
+
+#include "team_common.h"
+
+namespace cbk {
+namespace team {
+
+template <int N> constexpr int rows_per_lane() { return (N + TS - 1) / TS; }
+
+/* LDS words a team needs for the L copy: the packed lower triangle
+ * (L(i,c) at i(i+1)/2 + c), a 0.0 slot and a write sink */
+template <int N> constexpr int chol_lds() { return N * (N + 1) / 2 + 2; }
+
+/* Factor in registers. Returns the LAPACK-style code (first pivot <= 0 or NaN
+ * for Cholesky, first exact zero for LDL^T, 1-based; 0 if none). dg receives
+ * L(i,i) (Cholesky) or D(i) (LDL^T) for the lane's rows; the pivot entries of
+ * a become scratch. */
+template <class T, int N, bool LDL>
+__device__ __forceinline__ int chol_factor(T (&a)[rows_per_lane<N>()][N],
+                                           T (&dg)[rows_per_lane<N>()], int t)
+{
+    constexpr int R = rows_per_lane<N>();
+    int linfo = 0;
+    sfor<0, R>([&](auto r) { dg[r] = T(1); });
+    sfor<0, N>([&](auto jc) {
+        constexpr int j = jc;
+        constexpr int rj = j / TS, tj = j % TS;
+        constexpr int pe = cmin((j / 8) * 8 + 8, N);
+        const T piv = bcast<tj>(a[rj][j]);
+        if (linfo == 0 && (LDL ? (piv == T(0)) : !(piv > T(0)))) linfo = j + 1;
+        const T inv = T(1) / piv;
+        if constexpr (j + 1 < pe) {
+            /* panel form: A(i,c) -= A(i,j) * (A(c,j) * (1/a)) */
+            T wv[R];
+            sfor<0, R>([&](auto r) {
+                if constexpr (j < TS * (r + 1)) wv[r] = a[r][j] * inv;
+            });
+            sfor<j + 1, pe>([&](auto c) {
+                const T w = bcast<c % TS>(wv[c / TS]);
+                sfor<0, R>([&](auto r) {
+                    if constexpr (c < TS * (r + 1)) a[r][c] = fmaT(-a[r][j], w, a[r][c]);
+                });
+            });
+        }
+        /* scale the column below the pivot; the pivot's entry becomes scratch */
+        T sc;
+        if constexpr (LDL) {
+            sc = inv;
+            dg[rj] = (t == tj) ? piv : dg[rj];
+        }
+        else {
+            const T d = sqrtT(piv);
+            sc = d * inv;
+            dg[rj] = (t == tj) ? d : dg[rj];
+        }
+        sfor<0, R>([&](auto r) {
+            if constexpr (j < TS * (r + 1)) a[r][j] = a[r][j] * sc;
+        });
+        if constexpr (pe < N) {
+            /* trailing form: A(i,c) -= L(i,j) * w, w = L(c,j) or L(c,j)*D(j) */
+            T wd[R];
+            if constexpr (LDL) {
+                sfor<0, R>([&](auto r) {
+                    if constexpr (j < TS * (r + 1)) wd[r] = a[r][j] * piv;
+                });
+            }
+            sfor<pe, N>([&](auto c) {
+                const T w = LDL ? bcast<c % TS>(wd[c / TS]) : bcast<c % TS>(a[c / TS][j]);
+                sfor<0, R>([&](auto r) {
+                    if constexpr (c < TS * (r + 1)) a[r][c] = fmaT(-a[r][j], w, a[r][c]);
+                });
+            });
+        }
+        __builtin_amdgcn_sched_barrier(0); /* one pivot at a time: bounds live registers */
+    });
+    return linfo;
+}
+
+/* Ready a factor for chol_solve: dinv = 1/L(i,i) or 1/D(i) (sytrsnp's
+ * reciprocal, src/cbk_sytrsnp_compact.hpp:71); the diagonal and everything
+ * right of it zeroed (no row masks in the sweeps); L copied into the team's
+ * LDS (rows pre-scaled by 1/L(i,i) for potrs); for potrs the columns of a
+ * scaled by 1/L(c,c). */
+template <class T, int N, bool LDL>
+__device__ __forceinline__ void chol_prepare(T (&a)[rows_per_lane<N>()][N],
+                                             const T (&dg)[rows_per_lane<N>()],
+                                             T (&dinv)[rows_per_lane<N>()], T *L, int t)
+{
+    constexpr int R = rows_per_lane<N>();
+    constexpr int ZERO = N * (N + 1) / 2, SINK = ZERO + 1;
+    sfor<0, R>([&](auto r) { dinv[r] = T(1) / dg[r]; });
+    sfor<0, R>([&](auto r) {
+        const int i = t + TS * r;
+        sfor<TS * r, cmin(TS * (r + 1), N)>([&](auto c) { a[r][c] = (c < i) ? a[r][c] : T(0); });
+    });
+    sfor<0, R>([&](auto r) {
+        const int i = t + TS * r;
+        sfor<0, cmin(TS * (r + 1), N)>([&](auto c) {
+            const int at = ((i < N) && (c < TS * r || c < i)) ? i * (i + 1) / 2 + c : SINK;
+            L[at] = LDL ? a[r][c] : a[r][c] * dinv[r];
+        });
+    });
+    if (t == 0) L[ZERO] = T(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if constexpr (!LDL) {
+        sfor<0, N>([&](auto c) {
+            const T s = bcast<c % TS>(dinv[c / TS]);
+            sfor<0, R>([&](auto r) {
+                if constexpr (c < TS * (r + 1)) a[r][c] = a[r][c] * s;
+            });
+        });
+    }
+}
+
+/* X := A^{-1} B in place on b (the lane's rows, NRB right-hand sides) */
+template <class T, int N, int NRB, bool LDL>
+__device__ __forceinline__ void chol_solve(const T (&a)[rows_per_lane<N>()][N],
+                                           const T (&dinv)[rows_per_lane<N>()],
+                                           T (&b)[rows_per_lane<N>()][NRB], const T *L, int t)
+{
+    constexpr int R = rows_per_lane<N>();
+    constexpr int ZERO = N * (N + 1) / 2;
+    /* forward: L y = b */
+    sfor<0, N>([&](auto ic) {
+        constexpr int i = ic;
+        constexpr int ri = i / TS, ti = i % TS;
+        sfor<0, NRB>([&](auto q) {
+            const T y = bcast<ti>(b[ri][q]);
+            sfor<ri, R>([&](auto r) { b[r][q] = fmaT(-a[r][i], y, b[r][q]); });
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    /* potrs: x = y / L(i,i); sytrsnp: w = D^{-1} z */
+    sfor<0, R>([&](auto r) {
+        sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * dinv[r]; });
+    });
+    /* backward: L^T x = z, column-oriented from the bottom row */
+    sfor_down<0, N>([&](auto lc) {
+        constexpr int l = lc;
+        constexpr int rl = l / TS, tl = l % TS;
+        T lv[R];
+        sfor<0, rl + 1>([&](auto r) {
+            const int i = t + TS * r;
+            lv[r] = L[(i < l) ? l * (l + 1) / 2 + i : ZERO];
+        });
+        sfor<0, NRB>([&](auto q) {
+            const T z = bcast<tl>(b[rl][q]);
+            sfor<0, rl + 1>([&](auto r) { b[r][q] = fmaT(-lv[r], z, b[r][q]); });
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    if constexpr (!LDL) {
+        sfor<0, R>([&](auto r) {
+            sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * dinv[r]; });
+        });
+    }
+}
+
+/* max over the 16 lanes of the team, in every lane (mirror butterfly) */
+template <class T> __device__ __forceinline__ T team_max(T x)
+{
+    x = fmax_nan(x, dpp<0xB1>(x));  /* quad_perm [1,0,3,2] */
+    x = fmax_nan(x, dpp<0x4E>(x));  /* quad_perm [2,3,0,1] */
+    x = fmax_nan(x, dpp<0x141>(x)); /* row_half_mirror */
+    x = fmax_nan(x, dpp<0x140>(x)); /* row_mirror */
+    return x;
+}
+
+} /* namespace team */
+} /* namespace cbk */
+
+#endif /* CBK_TEAM_CHOL_H */

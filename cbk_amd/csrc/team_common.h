@@ -52,16 +52,45 @@ template <int S> __device__ __forceinline__ double bcast(double x)
     return __builtin_bit_cast(double, w);
 }
 
+/* value of another lane of the row under a DPP pattern (quad_perm, mirrors) */
+template <int CTRL> __device__ __forceinline__ float dpp(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL,
+                                                              0xF, 0xF, true));
+}
+template <int CTRL> __device__ __forceinline__ double dpp(double x)
+{
+    int2 w = __builtin_bit_cast(int2, x);
+    w.x = __builtin_amdgcn_mov_dpp(w.x, CTRL, 0xF, 0xF, true);
+    w.y = __builtin_amdgcn_mov_dpp(w.y, CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, w);
+}
+
+/* max that propagates NaN (a NaN residual must never look converged) */
+template <class T> __device__ __forceinline__ T fmax_nan(T a, T b)
+{
+    return (a > b || a != a) ? a : b;
+}
+
 /* ---- buffer addressing --------------------------------------------------
  * A team kernel addresses one compact group through a buffer resource whose
  * base is the group (wave-uniform: the 4 teams of a wave are 4 consecutive
  * slots of one group, V being a multiple of 4), a 32-bit per-lane byte offset
  * (row and slot) and a scalar byte offset (column): one VGPR per row block
  * instead of a 64-bit address per element. */
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t group_rsrc(const void *base)
+constexpr unsigned kOOB = 0x80000000u; /* a byte offset past every group's range */
+
+/* the byte range a group's resource covers: the rest of the buffer, capped so
+ * that kOOB stays out of range (a load there reads 0, a store is dropped) */
+__device__ __forceinline__ unsigned group_range(long groups_left, size_t gstride, unsigned sz)
 {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, 0xffffffff,
-                                             0x00020000);
+    const unsigned long long left = (unsigned long long)groups_left * gstride * sz;
+    return left < 0x7fffffffull ? (unsigned)left : 0x7fffffffu;
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t group_rsrc(const void *base, unsigned range)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, range, 0x00020000);
 }
 
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));

@@ -10,6 +10,8 @@
 
 #include "cbk.h"
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -66,6 +68,32 @@ int dgels_compact_info(char layout, char trans, int m, int n, int nrhs, double *
 int sgels_compact_info(char layout, char trans, int m, int n, int nrhs, float *ap,
                        int ldap, float *bp, int ldbp, float *taup, int V, int nm,
                        int *info);
+
+/* ---- mixed precision ----------------------------------------------------
+ * dsposv_compact: the compact, per-matrix form of LAPACK dsposv. Factor in
+ * FP32, solve, then refine X with residuals r = B - A X and updates X += d
+ * computed in FP64 until every column satisfies
+ *     ||r||_inf <= ||x||_inf * ||A||_inf * eps * sqrt(n)   (eps = 2^-53)
+ * or itermax = 30 steps pass; a matrix whose FP32 factor fails, whose data
+ * overflow FP32, or whose refinement does not converge is solved by the FP64
+ * Cholesky instead. Per matrix (device int[nm] each):
+ *     iter  >= 0 refinement steps taken (A untouched);
+ *           -2 overflow, -3 FP32 factor failed, -31 no convergence: the FP64
+ *           path ran and A holds its Cholesky factor;
+ *     info  0, or the FP64 dpotrf code of a matrix that is not SPD (X then
+ *           unwritten).
+ * B is read-only; X (n x nrhs, ldxp, same layout) receives the solution.
+ * work: device scratch of dsposv_compact_worksize(...) bytes (0 -- and work
+ * may be NULL -- when the fused kernel serves the shape: n in {16, 24, 32},
+ * nrhs <= 4, V >= 4; see that function).
+ * Codes: -1 layout -2 uplo -3 n -4 nrhs -6 ldap -8 ldbp -10 ldxp -12 lwork
+ *        -15 V -16 nm */
+int dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
+                   const double *bp, int ldbp, double *xp, int ldxp, void *work,
+                   size_t lwork, int *iter, int *info, int V, int nm);
+
+/* bytes of workspace dsposv_compact needs for this call shape (0 if none) */
+size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int V, int nm);
 
 #ifdef __cplusplus
 }

@@ -849,12 +849,22 @@ ORACLE_DEFS(float, s)
  * refinement steps taken; -3 the single factor failed; -31 refinement did not
  * converge in 30 steps. In both negative cases the double dposv ran. */
 int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
-                          const double *bp, int ldbp, double *xp, int ldxp, int *iter,
-                          int *info, int V, int nm)
+                          const double *bp, int ldbp, double *xp, int ldxp, void *work,
+                          size_t lwork, int *iter, int *info, int V, int nm)
 {
-    bool row, up;
-    if (int e = sym_solve_args(layout, uplo, n, nrhs, ldap, ldbp, V, nm, row, up)) return e;
+    (void)work;
+    (void)lwork;
+    const bool col = opt(layout, 'C'), row = opt(layout, 'R');
+    const bool lo = opt(uplo, 'L'), up = opt(uplo, 'U');
+    if (!col && !row) return -1;
+    if (!lo && !up) return -2;
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (ldap < max1(n)) return -6;
+    if (ldbp < max1(row ? nrhs : n)) return -8;
     if (ldxp < max1(row ? nrhs : n)) return -10;
+    if (!vlen_ok(V)) return -15;
+    if (nm < 0) return -16;
     for (int i = 0; i < nm; ++i) {
         iter[i] = 0;
         info[i] = 0;

@@ -14,6 +14,8 @@
 #ifndef CBK_ORACLE_H
 #define CBK_ORACLE_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -77,8 +79,8 @@ ORACLE_DECLS(float, s)
  * dposv otherwise. xp receives X; ap keeps A unless the fallback ran, in which
  * case it holds the double Cholesky factor. iter and info are per matrix. */
 int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
-                          const double *bp, int ldbp, double *xp, int ldxp, int *iter,
-                          int *info, int V, int nm);
+                          const double *bp, int ldbp, double *xp, int ldxp, void *work,
+                          size_t lwork, int *iter, int *info, int V, int nm);
 
 #ifdef __cplusplus
 }

@@ -165,6 +165,22 @@ class CLib:
             self._fns[key] = (abi.bind(self.lib, name, kinds, scalar), kinds)
         return self._fns[key]
 
+    def call_raw(self, name, *args):
+        """an include/cbk_ext.h entry point listed in abi.EXT, numpy buffers"""
+        key = ("ext", name)
+        if key not in self._fns:
+            kinds = abi.EXT[name]
+            self._fns[key] = (abi.bind(self.lib, self.prefix + name, kinds, "d"), kinds)
+        f, kinds = self._fns[key]
+        assert len(args) == len(kinds), (name, len(args), len(kinds))
+        cargs = []
+        for k, a in zip(kinds, args):
+            if k == "p":
+                cargs.append(None if a is None else a.ctypes.data_as(ctypes.c_void_p))
+            else:
+                cargs.append(abi.to_c(k, a))
+        return f(*cargs)
+
     def call(self, scalar, routine, *args, info=None):
         """Call with numpy arrays for the buffers (passed by pointer, in place)."""
         f, kinds = self.fn(scalar, routine, info is not None)
