@@ -48,12 +48,14 @@ struct MixedArgs {
     int *info;
 };
 
+/* 4 teams (one wavefront) per block: at one wavefront per SIMD a 4-wave
+ * block would hold its CU until its slowest matrix converged */
 constexpr int kTeamsPerBlock = 4;
 
 template <int N> hipError_t launch_n(const MixedArgs &args);
 
 template <int N, int NRB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
 mixed_posv_kernel(MixedArgs p)
 {
     constexpr int R = rows_per_lane<N>();
@@ -61,12 +63,13 @@ mixed_posv_kernel(MixedArgs p)
     __shared__ double lds[kTeamsPerBlock * chol_lds<N>()];
 
     const int t = threadIdx.x & (TS - 1);
-    const int team = threadIdx.x / TS;
-    const long mat0 = (long)blockIdx.x * kTeamsPerBlock;
-    const long mat = mat0 + team;
+    const int team = threadIdx.x / TS;                    /* 0..15 in the block */
+    const long mat0 = (long)blockIdx.x * kTeamsPerBlock + (team & ~3); /* wave-uniform */
+    if (mat0 >= p.nmat) return;                           /* whole wavefront */
+    const long mat = mat0 + (team & 3);
     const long g = mat0 / p.V;
     const long groups_left = p.nmat / p.V - g;
-    const unsigned v = (unsigned)(mat0 % p.V) + team;
+    const unsigned v = (unsigned)(mat0 % p.V) + (team & 3);
     const unsigned V = (unsigned)p.V;
     constexpr unsigned SZ = 8;
     const __amdgpu_buffer_rsrc_t ra = group_rsrc(p.a + g * p.a_gs, group_range(groups_left, p.a_gs, SZ));
@@ -150,8 +153,7 @@ mixed_posv_kernel(MixedArgs p)
                 sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { res[r][q] = b64[r][q]; }); });
                 sfor<0, N>([&](auto c) {
                     sfor<0, NRB>([&](auto q) {
-                        const double xc = bcast<c % TS>(x64[c / TS][q]);
-                        sfor<0, R>([&](auto r) { res[r][q] = fmaT(-a64[r][c], xc, res[r][q]); });
+                        sfor<0, R>([&](auto r) { fnma_bc<c % TS>(res[r][q], x64[c / TS][q], a64[r][c]); });
                     });
                 });
                 /* every column: ||r||_inf <= ||x||_inf * cte (NaN fails) */

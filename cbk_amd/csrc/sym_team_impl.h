@@ -57,24 +57,27 @@ template <class T> struct SymArgs {
     bool factor;
 };
 
-constexpr int kTeamsPerBlock = 4; /* one wavefront per block */
+/* 16 teams (4 wavefronts) per block: at V >= 16 a block's matrices fill whole
+ * 128-byte lines of every element, so each line is fetched by one CU */
+constexpr int kTeamsPerBlock = 16;
 
 template <class T, int N> hipError_t launch_n(bool ldl, const SymArgs<T> &args);
 
 template <class T, int N, int NRB, bool LDL>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
 sym_team_kernel(SymArgs<T> p)
 {
     constexpr int R = rows_per_lane<N>();
     __shared__ T lds[kTeamsPerBlock * chol_lds<N>()];
 
     const int t = threadIdx.x & (TS - 1);
-    const int team = threadIdx.x / TS;
-    const long mat0 = (long)blockIdx.x * kTeamsPerBlock; /* wave-uniform */
-    const long mat = mat0 + team;
+    const int team = threadIdx.x / TS;                    /* 0..15 in the block */
+    const long mat0 = (long)blockIdx.x * kTeamsPerBlock + (team & ~3); /* wave-uniform */
+    if (mat0 >= p.nmat) return;                           /* whole wavefront */
+    const long mat = mat0 + (team & 3);
     const long g = mat0 / p.V;
     const long groups_left = p.nmat / p.V - g;
-    const unsigned v = (unsigned)(mat0 % p.V) + team;
+    const unsigned v = (unsigned)(mat0 % p.V) + (team & 3);
     const unsigned V = (unsigned)p.V;
     constexpr unsigned SZ = sizeof(T);
     const __amdgpu_buffer_rsrc_t ra = group_rsrc(p.a + g * p.a_gs, group_range(groups_left, p.a_gs, SZ));
@@ -91,7 +94,11 @@ sym_team_kernel(SymArgs<T> p)
     T dg[R]; /* the diagonal of the factor for the lane's rows: L(i,i) or D(i) */
     int linfo = 0;
     sfor<0, R>([&](auto r) {
-        sfor<0, cmin(TS * (r + 1), N)>([&](auto c) { a[r][c] = bload(T(), ra, a_row[r], c * a_col); });
+        const int i = t + TS * r;
+        /* the lower triangle only: entries right of the diagonal read as 0 */
+        sfor<0, cmin(TS * (r + 1), N)>([&](auto c) {
+            a[r][c] = bload(T(), ra, (c < TS * r || c <= i) ? a_row[r] : kOOB, c * a_col);
+        });
     });
 
     if (p.factor) {
@@ -149,7 +156,7 @@ sym_team_kernel(SymArgs<T> p)
 template <class T, int N> hipError_t launch_n(bool ldl, const SymArgs<T> &args)
 {
     const unsigned blocks = (unsigned)((args.nmat + kTeamsPerBlock - 1) / kTeamsPerBlock);
-    const dim3 grid(blocks), block(64);
+    const dim3 grid(blocks), block(256);
     if (args.nrhs > 4) {
         if (ldl)
             hipLaunchKernelGGL((sym_team_kernel<T, N, 8, true>), grid, block, 0, current_stream(), args);

@@ -62,9 +62,8 @@ __device__ __forceinline__ int chol_factor(T (&a)[rows_per_lane<N>()][N],
                 if constexpr (j < TS * (r + 1)) wv[r] = a[r][j] * inv;
             });
             sfor<j + 1, pe>([&](auto c) {
-                const T w = bcast<c % TS>(wv[c / TS]);
                 sfor<0, R>([&](auto r) {
-                    if constexpr (c < TS * (r + 1)) a[r][c] = fmaT(-a[r][j], w, a[r][c]);
+                    if constexpr (c < TS * (r + 1)) fnma_bc<c % TS>(a[r][c], wv[c / TS], a[r][j]);
                 });
             });
         }
@@ -91,9 +90,9 @@ __device__ __forceinline__ int chol_factor(T (&a)[rows_per_lane<N>()][N],
                 });
             }
             sfor<pe, N>([&](auto c) {
-                const T w = LDL ? bcast<c % TS>(wd[c / TS]) : bcast<c % TS>(a[c / TS][j]);
                 sfor<0, R>([&](auto r) {
-                    if constexpr (c < TS * (r + 1)) a[r][c] = fmaT(-a[r][j], w, a[r][c]);
+                    if constexpr (c < TS * (r + 1))
+                        fnma_bc<c % TS>(a[r][c], LDL ? wd[c / TS] : a[c / TS][j], a[r][j]);
                 });
             });
         }
@@ -153,8 +152,7 @@ __device__ __forceinline__ void chol_solve(const T (&a)[rows_per_lane<N>()][N],
         constexpr int i = ic;
         constexpr int ri = i / TS, ti = i % TS;
         sfor<0, NRB>([&](auto q) {
-            const T y = bcast<ti>(b[ri][q]);
-            sfor<ri, R>([&](auto r) { b[r][q] = fmaT(-a[r][i], y, b[r][q]); });
+            sfor<ri, R>([&](auto r) { fnma_bc<ti>(b[r][q], b[ri][q], a[r][i]); });
         });
         __builtin_amdgcn_sched_barrier(0);
     });
@@ -172,8 +170,7 @@ __device__ __forceinline__ void chol_solve(const T (&a)[rows_per_lane<N>()][N],
             lv[r] = L[(i < l) ? l * (l + 1) / 2 + i : ZERO];
         });
         sfor<0, NRB>([&](auto q) {
-            const T z = bcast<tl>(b[rl][q]);
-            sfor<0, rl + 1>([&](auto r) { b[r][q] = fmaT(-lv[r], z, b[r][q]); });
+            sfor<0, rl + 1>([&](auto r) { fnma_bc<tl>(b[r][q], b[rl][q], lv[r]); });
         });
         __builtin_amdgcn_sched_barrier(0);
     });

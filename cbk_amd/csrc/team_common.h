@@ -52,6 +52,24 @@ template <int S> __device__ __forceinline__ double bcast(double x)
     return __builtin_bit_cast(double, w);
 }
 
+/* acc := fma(-x_S, a, acc), with x_S the value of x in lane S of this lane's
+ * row: the broadcast rides on the fma as its DPP source operand
+ * (v_fmac_*_dpp row_newbcast), one VALU op instead of a move and an fma. The
+ * s_nop covers the two wait states a DPP read needs after the VALU write of
+ * its source. Same rounding as fma(-a, x_S, acc). */
+template <int S> __device__ __forceinline__ void fnma_bc(double &acc, double x, double a)
+{
+    asm("s_nop 1\n\tv_fmac_f64_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(x), "v"(a), "i"(S));
+}
+template <int S> __device__ __forceinline__ void fnma_bc(float &acc, float x, float a)
+{
+    asm("s_nop 1\n\tv_fmac_f32_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(x), "v"(a), "i"(S));
+}
+
 /* value of another lane of the row under a DPP pattern (quad_perm, mirrors) */
 template <int CTRL> __device__ __forceinline__ float dpp(float x)
 {
