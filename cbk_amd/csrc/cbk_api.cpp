@@ -64,7 +64,7 @@ template <class T> int geqrf(char layout, int m, int n, T *ap, int ldap, T *taup
     if (!vlen_ok(V)) return -7;
     if (nm < 0) return -8;
     if (m == 0 || n == 0 || nm == 0) return 0;
-    return status(launch_geqrf_lane<T>(m, n, operand(ap, ldap, row, m, n, V), taup, V, nm));
+    return status(dispatch_geqrf<T>(m, n, operand(ap, ldap, row, m, n, V), taup, V, nm));
 }
 
 /* src/cbk.cpp:80-105 */

@@ -76,6 +76,15 @@ hipError_t launch_mixed_posv_team(bool up, int n, int nrhs, Operand<double> A,
                                   Operand<double> B, Operand<double> X, int V, int nm,
                                   int itermax, int *iter, int *info);
 
+/* shapes served by the QR team kernel: the tall view p x q (p >= q) */
+bool qr_team_supports(int p, int q, int nrhs, bool fp64);
+
+/* geqrf (B.p == null) or overdetermined gels on the tall view of A
+ * (transposed: the view is A^T); tau receives q scalars per matrix */
+template <class T>
+hipError_t launch_qr_team(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
+                          Operand<T> B, int V, int nm, int *info);
+
 /* B(0:rows, 0:cols) := 0 in every group */
 template <class T>
 hipError_t launch_zero(int rows, int cols, Operand<T> B, int V, int nm);

@@ -26,7 +26,23 @@ template <class T>
 hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operand<T> B,
                          T *tau, int V, int nm, int *info)
 {
+    const bool tall = m >= n;
+    const int p = tall ? m : n, q = tall ? n : m;
+    const bool overdet = (tall != tran); /* src/cbk_gels_compact.hpp:124 */
+    if (current_policy() == CBK_POLICY_AUTO && overdet && V >= 4 &&
+        qr_team_supports(p, q, nrhs, sizeof(T) == 8))
+        return launch_qr_team<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
     return launch_gels_lane<T>(tran, m, n, nrhs, A, B, tau, V, nm, info);
+}
+
+template <class T> hipError_t dispatch_geqrf(int m, int n, Operand<T> A, T *tau, int V, int nm)
+{
+    if (current_policy() == CBK_POLICY_AUTO && m >= n && V >= 4 &&
+        qr_team_supports(m, n, 0, sizeof(T) == 8)) {
+        const Operand<T> none = {nullptr, 1, A.row, 0};
+        return launch_qr_team<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
+    }
+    return launch_geqrf_lane<T>(m, n, A, tau, V, nm);
 }
 
 #define CBK_INSTANTIATE_DISPATCH(T)                                                      \
@@ -35,7 +51,8 @@ hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operan
     template hipError_t dispatch_trsm<T>(bool, bool, bool, bool, int, int, T, Operand<T>, \
                                          Operand<T>, int, int);                           \
     template hipError_t dispatch_gels<T>(bool, int, int, int, Operand<T>, Operand<T>,     \
-                                         T *, int, int, int *);
+                                         T *, int, int, int *);                           \
+    template hipError_t dispatch_geqrf<T>(int, int, Operand<T>, T *, int, int);
 
 CBK_INSTANTIATE_DISPATCH(double)
 CBK_INSTANTIATE_DISPATCH(float)

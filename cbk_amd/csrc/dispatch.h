@@ -21,6 +21,8 @@ template <class T>
 hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operand<T> B,
                          T *tau, int V, int nm, int *info);
 
+template <class T> hipError_t dispatch_geqrf(int m, int n, Operand<T> A, T *tau, int V, int nm);
+
 } /* namespace cbk */
 
 #endif /* CBK_DISPATCH_H */

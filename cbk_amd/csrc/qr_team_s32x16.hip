@@ -1,0 +1,8 @@
+/* the QR / least-squares team kernel for 32 x 16 matrices, float */
+#include "qr_team_impl.h"
+
+namespace cbk {
+namespace qrteam {
+template hipError_t launch_pq<float, 32, 16, 2>(const QrArgs<float> &);
+}
+} /* namespace cbk */
