@@ -11,7 +11,7 @@
  * divides, square roots and multiplies, and an fma() wherever the reference's
  * compiler fuses a multiply-add (the library is built without contraction, so
  * nothing else fuses). Results are therefore bit-identical to the reference
- * kernels. The fast team kernels (posv_team.hip and others) take over the hot
+ * kernels. The team kernels (sym_team_impl.h, qr_team_impl.h, mixed_team_impl.h) take the hot
  * shapes; this file is the fallback that covers everything.
  */
 
