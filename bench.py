@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nm", type=int, default=NM)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a plain 1 GiB device-to-device copy (the known-size kernel "
+                         "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -52,23 +55,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_inputs(torch, nm, seed=2026):
-    """dense FP64 A (nm, n, n) and B (nm, n, nrhs) on the GPU"""
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    A = torch.empty(nm, N, N, dtype=torch.float64, device="cuda")
+def make_inputs(nm, seed=2026):
+    """dense FP64 A (nm, n, n) and B (nm, n, nrhs) on the host (numpy): A = Q
+    diag(lambda) Q^T with Q from the QR of an N(0,1) matrix (sign-fixed so
+    diag(R) > 0), log10(lambda) ~ U[0, k], k from {2, 4, 6} per matrix,
+    symmetrised; B ~ N(0, 1)"""
+    import numpy as np
+    r = np.random.default_rng(seed)
+    A = np.empty((nm, N, N))
     chunk = 16384
     for s in range(0, nm, chunk):
         e = min(nm, s + chunk)
-        G = torch.randn(e - s, N, N, dtype=torch.float64, device="cuda", generator=g)
-        Q, Rr = torch.linalg.qr(G)
-        Q = Q * torch.sign(torch.diagonal(Rr, dim1=1, dim2=2)).unsqueeze(1)
-        k = torch.tensor([2.0, 4.0, 6.0], dtype=torch.float64, device="cuda")[
-            torch.randint(0, 3, (e - s,), device="cuda", generator=g)]
-        u = torch.rand(e - s, N, dtype=torch.float64, device="cuda", generator=g)
-        lam = torch.pow(10.0, u * k[:, None])
-        M = torch.matmul(Q * lam.unsqueeze(1), Q.transpose(1, 2).contiguous())
-        A[s:e] = 0.5 * (M + M.transpose(1, 2))
-    B = torch.randn(nm, N, NRHS, dtype=torch.float64, device="cuda", generator=g)
+        G = r.standard_normal((e - s, N, N))
+        Q, Rr = np.linalg.qr(G)
+        Q = Q * np.sign(np.diagonal(Rr, axis1=1, axis2=2))[:, None, :]
+        k = np.array([2.0, 4.0, 6.0])[r.integers(0, 3, e - s)]
+        lam = 10.0 ** (r.uniform(0, 1, (e - s, N)) * k[:, None])
+        M = (Q * lam[:, None, :]) @ np.swapaxes(Q, 1, 2)
+        A[s:e] = 0.5 * (M + np.swapaxes(M, 1, 2))
+    B = r.standard_normal((nm, N, NRHS))
     return A, B
 
 
@@ -97,13 +102,26 @@ def backward_error(torch, A, X, B):
     return (num / den).amax(dim=1)
 
 
-def cpu_baseline(nm):
+def load_traffic():
+    """HBM bytes per launch of the two kernels, measured with rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE on this benchmark and scaled by the same counters on
+    a plain copy of known size (profiles/dposv_k4_traffic.json), or None"""
+    path = os.path.join(ROOT, "profiles", "dposv_k4_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def cpu_baseline(A, B):
     """the reference's dposv_compact (oracle/_ref) on this host's cores, FP64,
-    n=32, nrhs=4, V=8, the same synthetic distribution; best of 3 passes with an
-    untimed restore (the reference's best_time, examples/bench_util.hpp:250-267)"""
+    n=32, nrhs=4, V=8 (the AVX-512 width), on the benchmark's own matrices;
+    best of 3 passes with an untimed restore (the reference's best_time,
+    examples/bench_util.hpp:250-267)"""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import cbkt
+    nm = A.shape[0]
     path = cbkt.ref_path()
     kind = "reference"
     if path is None:
@@ -114,15 +132,6 @@ def cpu_baseline(nm):
     fn.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                    ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     fn.restype = ctypes.c_int
-    r = np.random.default_rng(7)
-    G = r.standard_normal((nm, N, N))
-    Q, Rr = np.linalg.qr(G)
-    Q = Q * np.sign(np.diagonal(Rr, axis1=1, axis2=2))[:, None, :]
-    k = np.array([2.0, 4.0, 6.0])[r.integers(0, 3, nm)]
-    lam = 10.0 ** (r.uniform(0, 1, (nm, N)) * k[:, None])
-    A = (Q * lam[:, None, :]) @ np.swapaxes(Q, 1, 2)
-    A = 0.5 * (A + np.swapaxes(A, 1, 2))
-    B = r.standard_normal((nm, N, NRHS))
     Vc = 8
     ap0 = cbkt.pack(A, Vc, fill=0.0)
     bp0 = cbkt.pack(B, Vc, pad_diag=0.0, fill=0.0)
@@ -165,7 +174,9 @@ def main():
     K, W = args.steps, args.warmup
 
     log(f"generating {nm} SPD matrices n={N} (cond up to 1e6) ...")
-    A, B = make_inputs(torch, nm)
+    A_host, B_host = make_inputs(nm)
+    A = torch.from_numpy(A_host).cuda()
+    B = torch.from_numpy(B_host).cuda()
     Ap, Bp = pack(torch, A, V), pack(torch, B, V)
 
     # ---- plain FP64 dposv: one pre-filled copy per step
@@ -268,9 +279,18 @@ def main():
             "bytes_per_matrix": mixed_bytes,
         },
     }
+    traffic = load_traffic()
+    if traffic:
+        result["roofline"]["traffic"] = traffic["mixed_posv_kernel"]["bytes_per_launch"]
+        result["plain_fp64"]["traffic_bytes_per_launch"] = traffic["sym_team_kernel"]["bytes_per_launch"]
+    if args.calibrate:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     if not args.no_cpu:
         log("CPU baseline (reference dposv_compact on the host) ...")
-        result["cpu_baseline"] = cpu_baseline(nm)
+        result["cpu_baseline"] = cpu_baseline(A_host, B_host)
     print(json.dumps(result), flush=True)
 
 

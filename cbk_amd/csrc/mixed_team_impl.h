@@ -64,7 +64,7 @@ mixed_posv_kernel(MixedArgs p)
 
     const int t = threadIdx.x & (TS - 1);
     const int team = threadIdx.x / TS;                    /* 0..15 in the block */
-    const long mat0 = (long)blockIdx.x * kTeamsPerBlock + (team & ~3); /* wave-uniform */
+    const long mat0 = (long)xcd_block(blockIdx.x) * kTeamsPerBlock + (team & ~3);
     if (mat0 >= p.nmat) return;                           /* whole wavefront */
     const long mat = mat0 + (team & 3);
     const long g = mat0 / p.V;
@@ -238,7 +238,8 @@ mixed_posv_kernel(MixedArgs p)
 template <int N> hipError_t launch_n(const MixedArgs &args)
 {
     const unsigned blocks = (unsigned)((args.nmat + kTeamsPerBlock - 1) / kTeamsPerBlock);
-    hipLaunchKernelGGL((mixed_posv_kernel<N, 4>), dim3(blocks), dim3(64), 0, current_stream(), args);
+    hipLaunchKernelGGL((mixed_posv_kernel<N, 4>), dim3(xcd_grid(blocks)), dim3(64), 0,
+                       current_stream(), args);
     return hipGetLastError();
 }
 

@@ -70,6 +70,19 @@ template <int S> __device__ __forceinline__ void fnma_bc(float &acc, float x, fl
         : "v"(x), "v"(a), "i"(S));
 }
 
+/* XCD-aware block order. MI355X deals consecutive workgroups round-robin to
+ * its 8 XCDs, each with its own L2. Logical blocks 4k..4k+3 (whose matrices
+ * share 128-byte lines at V = 64 when a block holds 4 matrices) are mapped to
+ * one XCD's consecutive dispatch slots, so a line is fetched from HBM once.
+ * The grid is padded to a multiple of 32 blocks; padding blocks get a logical
+ * index >= nblocks and exit. */
+__device__ __forceinline__ unsigned xcd_block(unsigned phys)
+{
+    const unsigned xcd = phys & 7u, slot = phys >> 3;
+    return (slot >> 2) * 32u + xcd * 4u + (slot & 3u);
+}
+constexpr unsigned xcd_grid(unsigned nblocks) { return (nblocks + 31u) & ~31u; }
+
 /* value of another lane of the row under a DPP pattern (quad_perm, mirrors) */
 template <int CTRL> __device__ __forceinline__ float dpp(float x)
 {
