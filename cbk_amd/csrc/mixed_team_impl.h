@@ -15,8 +15,9 @@
  * why (-2 overflow, -3 FP32 factor failed, -(ITERMAX+1) no convergence), as
  * dsposv's ITER does.
  *
- * The FP32 factor and solves are the team building blocks of team_chol.h; the
- * symmetric residual sums A(i,c) x_c over c ascending with one FP64 fma each.
+ * The FP32 factor (chol_factor_fast: hardware reciprocal square root) and the
+ * FP32 solves are the team building blocks of team_chol.h; the symmetric
+ * residual sums A(i,c) x_c over c ascending with one FP64 fma each.
  */
 #ifndef CBK_MIXED_TEAM_IMPL_H
 #define CBK_MIXED_TEAM_IMPL_H
@@ -64,7 +65,7 @@ mixed_posv_kernel(MixedArgs p)
 
     const int t = threadIdx.x & (TS - 1);
     const int team = threadIdx.x / TS;                    /* 0..15 in the block */
-    const long mat0 = (long)xcd_block(blockIdx.x) * kTeamsPerBlock + (team & ~3);
+    const long mat0 = (long)xcd_block(blockIdx.x) * kTeamsPerBlock + 4 * wave_in_block(); /* wave-uniform */
     if (mat0 >= p.nmat) return;                           /* whole wavefront */
     const long mat = mat0 + (team & 3);
     const long g = mat0 / p.V;
@@ -137,7 +138,7 @@ mixed_posv_kernel(MixedArgs p)
         sfor<0, R>([&](auto r) {
             sfor<0, cmin(TS * (r + 1), N)>([&](auto c) { a32[r][c] = (float)a64[r][c]; });
         });
-        const int finfo = chol_factor<float, N, false>(a32, dg32, t);
+        const int finfo = chol_factor_fast<N>(a32, dg32, t);
         if (finfo != 0) {
             iter = -3;
         }

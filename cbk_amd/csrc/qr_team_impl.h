@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void qr_team_kernel(QrArgs<T> p)
     constexpr unsigned SZ = sizeof(T);
     const int t = threadIdx.x & (TS - 1);
     const int team = threadIdx.x / TS;
-    const long mat0 = (long)blockIdx.x * kTeamsPerBlock + (team & ~3);
+    const long mat0 = (long)blockIdx.x * kTeamsPerBlock + 4 * wave_in_block(); /* wave-uniform */
     if (mat0 >= p.nmat) return;
     const long mat = mat0 + (team & 3);
     const long g = mat0 / p.V;

@@ -72,7 +72,7 @@ sym_team_kernel(SymArgs<T> p)
 
     const int t = threadIdx.x & (TS - 1);
     const int team = threadIdx.x / TS;                    /* 0..15 in the block */
-    const long mat0 = (long)blockIdx.x * kTeamsPerBlock + (team & ~3); /* wave-uniform */
+    const long mat0 = (long)blockIdx.x * kTeamsPerBlock + 4 * wave_in_block(); /* wave-uniform */
     if (mat0 >= p.nmat) return;                           /* whole wavefront */
     const long mat = mat0 + (team & 3);
     const long g = mat0 / p.V;

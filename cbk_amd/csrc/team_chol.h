@@ -41,6 +41,25 @@ template <int N> constexpr int chol_lds() { return N * (N + 1) / 2 + 2; }
  * for Cholesky, first exact zero for LDL^T, 1-based; 0 if none). dg receives
  * L(i,i) (Cholesky) or D(i) (LDL^T) for the lane's rows; the pivot entries of
  * a become scratch. */
+/* the pivot j as every lane of the team sees it: a, 1/a and (Cholesky) sqrt(a) */
+template <class T, bool LDL> struct Pivot {
+    T piv, inv, d;
+};
+template <class T, int N, bool LDL, int J>
+__device__ __forceinline__ Pivot<T, LDL> load_pivot(const T (&a)[rows_per_lane<N>()][N])
+{
+    Pivot<T, LDL> p;
+    p.piv = bcast<J % TS>(a[J / TS][J]);
+    p.inv = T(1) / p.piv;
+    p.d = LDL ? T(0) : sqrtT(p.piv);
+    return p;
+}
+
+/* Look-ahead: pivot j+1's divide and square root depend only on column j+1,
+ * so that column is updated first and the next pivot's chain is issued
+ * before the rest of pivot j's updates, which hide its latency. The
+ * operations on every element, and their order per element, are those
+ * described above. */
 template <class T, int N, bool LDL>
 __device__ __forceinline__ int chol_factor(T (&a)[rows_per_lane<N>()][N],
                                            T (&dg)[rows_per_lane<N>()], int t)
@@ -48,24 +67,45 @@ __device__ __forceinline__ int chol_factor(T (&a)[rows_per_lane<N>()][N],
     constexpr int R = rows_per_lane<N>();
     int linfo = 0;
     sfor<0, R>([&](auto r) { dg[r] = T(1); });
+    Pivot<T, LDL> cur = load_pivot<T, N, LDL, 0>(a);
     sfor<0, N>([&](auto jc) {
         constexpr int j = jc;
         constexpr int rj = j / TS, tj = j % TS;
         constexpr int pe = cmin((j / 8) * 8 + 8, N);
-        const T piv = bcast<tj>(a[rj][j]);
+        const T piv = cur.piv, inv = cur.inv;
         if (linfo == 0 && (LDL ? (piv == T(0)) : !(piv > T(0)))) linfo = j + 1;
-        const T inv = T(1) / piv;
-        if constexpr (j + 1 < pe) {
-            /* panel form: A(i,c) -= A(i,j) * (A(c,j) * (1/a)) */
-            T wv[R];
-            sfor<0, R>([&](auto r) {
-                if constexpr (j < TS * (r + 1)) wv[r] = a[r][j] * inv;
-            });
-            sfor<j + 1, pe>([&](auto c) {
+        Pivot<T, LDL> nxt = cur;
+        /* panel form on columns [c0, c1): A(i,c) -= A(i,j) * (A(c,j) * (1/a)) */
+        T wv[R];
+        auto panel = [&](auto c0, auto c1) {
+            sfor<c0, c1>([&](auto c) {
                 sfor<0, R>([&](auto r) {
                     if constexpr (c < TS * (r + 1)) fnma_bc<c % TS>(a[r][c], wv[c / TS], a[r][j]);
                 });
             });
+        };
+        /* trailing form on columns [c0, c1): A(i,c) -= L(i,j) * w, w = L(c,j) or L(c,j)*D(j) */
+        T wd[R];
+        auto trailing = [&](auto c0, auto c1) {
+            sfor<c0, c1>([&](auto c) {
+                sfor<0, R>([&](auto r) {
+                    if constexpr (c < TS * (r + 1))
+                        fnma_bc<c % TS>(a[r][c], LDL ? wd[c / TS] : a[c / TS][j], a[r][j]);
+                });
+            });
+        };
+        using I1 = std::integral_constant<int, j + 1>;
+        using I2 = std::integral_constant<int, j + 2>;
+        using IPE = std::integral_constant<int, pe>;
+        using IPE1 = std::integral_constant<int, pe + 1>;
+        using IN = std::integral_constant<int, N>;
+        if constexpr (j + 1 < pe) {
+            sfor<0, R>([&](auto r) {
+                if constexpr (j < TS * (r + 1)) wv[r] = a[r][j] * inv;
+            });
+            panel(I1{}, I2{});
+            nxt = load_pivot<T, N, LDL, j + 1>(a);
+            panel(I2{}, IPE{});
         }
         /* scale the column below the pivot; the pivot's entry becomes scratch */
         T sc;
@@ -74,29 +114,69 @@ __device__ __forceinline__ int chol_factor(T (&a)[rows_per_lane<N>()][N],
             dg[rj] = (t == tj) ? piv : dg[rj];
         }
         else {
-            const T d = sqrtT(piv);
-            sc = d * inv;
-            dg[rj] = (t == tj) ? d : dg[rj];
+            sc = cur.d * inv;
+            dg[rj] = (t == tj) ? cur.d : dg[rj];
         }
         sfor<0, R>([&](auto r) {
             if constexpr (j < TS * (r + 1)) a[r][j] = a[r][j] * sc;
         });
         if constexpr (pe < N) {
-            /* trailing form: A(i,c) -= L(i,j) * w, w = L(c,j) or L(c,j)*D(j) */
-            T wd[R];
             if constexpr (LDL) {
                 sfor<0, R>([&](auto r) {
                     if constexpr (j < TS * (r + 1)) wd[r] = a[r][j] * piv;
                 });
             }
-            sfor<pe, N>([&](auto c) {
-                sfor<0, R>([&](auto r) {
-                    if constexpr (c < TS * (r + 1))
-                        fnma_bc<c % TS>(a[r][c], LDL ? wd[c / TS] : a[c / TS][j], a[r][j]);
-                });
-            });
+            if constexpr (j + 1 == pe) {
+                trailing(IPE{}, IPE1{});
+                nxt = load_pivot<T, N, LDL, j + 1>(a);
+                trailing(IPE1{}, IN{});
+            }
+            else {
+                trailing(IPE{}, IN{});
+            }
         }
+        cur = nxt;
         __builtin_amdgcn_sched_barrier(0); /* one pivot at a time: bounds live registers */
+    });
+    return linfo;
+}
+
+/* FP32 Cholesky for the mixed-precision solve, where the factor only has to
+ * be an FP32-accurate preconditioner (refinement in FP64 removes its error):
+ * plain right-looking per pivot, L(:,j) = A(:,j) * rsq(a) with the hardware
+ * reciprocal square root, L(j,j) = a * rsq(a), then the trailing update with
+ * the scaled column. Same look-ahead as chol_factor. Returns the first pivot
+ * <= 0 or NaN (1-based), 0 if none. */
+template <int N>
+__device__ __forceinline__ int chol_factor_fast(float (&a)[rows_per_lane<N>()][N],
+                                                float (&dg)[rows_per_lane<N>()], int t)
+{
+    constexpr int R = rows_per_lane<N>();
+    int linfo = 0;
+    sfor<0, R>([&](auto r) { dg[r] = 1.0f; });
+    float piv = bcast<0>(a[0][0]);
+    float rs = __builtin_amdgcn_rsqf(piv);
+    sfor<0, N>([&](auto jc) {
+        constexpr int j = jc;
+        constexpr int rj = j / TS, tj = j % TS;
+        if (linfo == 0 && !(piv > 0.0f)) linfo = j + 1;
+        dg[rj] = (t == tj) ? piv * rs : dg[rj];
+        sfor<0, R>([&](auto r) {
+            if constexpr (j < TS * (r + 1)) a[r][j] = a[r][j] * rs;
+        });
+        float npiv = piv, nrs = rs;
+        sfor<j + 1, N>([&](auto c) {
+            sfor<0, R>([&](auto r) {
+                if constexpr (c < TS * (r + 1)) fnma_bc<c % TS>(a[r][c], a[c / TS][j], a[r][j]);
+            });
+            if constexpr (c == j + 1) {
+                npiv = bcast<c % TS>(a[c / TS][c]);
+                nrs = __builtin_amdgcn_rsqf(npiv);
+            }
+        });
+        piv = npiv;
+        rs = nrs;
+        __builtin_amdgcn_sched_barrier(0);
     });
     return linfo;
 }

@@ -70,6 +70,15 @@ template <int S> __device__ __forceinline__ void fnma_bc(float &acc, float x, fl
         : "v"(x), "v"(a), "i"(S));
 }
 
+/* Index of the calling wavefront in its block, as a value the compiler knows
+ * to be wave-uniform. Bases derived from it (group pointers, buffer
+ * resources) stay in SGPRs; derived from threadIdx.x directly they would be
+ * treated as divergent and every buffer access would become a waterfall loop. */
+__device__ __forceinline__ int wave_in_block()
+{
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 /* XCD-aware block order. MI355X deals consecutive workgroups round-robin to
  * its 8 XCDs, each with its own L2. Logical blocks 4k..4k+3 (whose matrices
  * share 128-byte lines at V = 64 when a block holds 4 matrices) are mapped to
