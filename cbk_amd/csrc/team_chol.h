@@ -240,18 +240,28 @@ __device__ __forceinline__ void chol_solve(const T (&a)[rows_per_lane<N>()][N],
     sfor<0, R>([&](auto r) {
         sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * dinv[r]; });
     });
-    /* backward: L^T x = z, column-oriented from the bottom row */
-    sfor_down<0, N>([&](auto lc) {
+    /* backward: L^T x = z, column-oriented from the bottom row; the LDS reads
+     * of row l-1 are issued before step l's fmas, which hide their latency */
+    auto load_row = [&](auto lc, T(&lv)[R]) {
         constexpr int l = lc;
-        constexpr int rl = l / TS, tl = l % TS;
-        T lv[R];
-        sfor<0, rl + 1>([&](auto r) {
+        sfor<0, l / TS + 1>([&](auto r) {
             const int i = t + TS * r;
             lv[r] = L[(i < l) ? l * (l + 1) / 2 + i : ZERO];
         });
+    };
+    T lv[R];
+    load_row(std::integral_constant<int, N - 1>{}, lv);
+    sfor_down<0, N>([&](auto lc) {
+        constexpr int l = lc;
+        constexpr int rl = l / TS, tl = l % TS;
+        T lvn[R];
+        if constexpr (l > 0) load_row(std::integral_constant<int, l - 1>{}, lvn);
         sfor<0, NRB>([&](auto q) {
             sfor<0, rl + 1>([&](auto r) { fnma_bc<tl>(b[r][q], b[rl][q], lv[r]); });
         });
+        if constexpr (l > 0) {
+            sfor<0, R>([&](auto r) { lv[r] = lvn[r]; });
+        }
         __builtin_amdgcn_sched_barrier(0);
     });
     if constexpr (!LDL) {
