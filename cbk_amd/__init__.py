@@ -146,3 +146,112 @@ def last_error():
 
 def version():
     return lib().cbk_get_version()
+
+
+# ---- uneven batches (include/cbk_ext.h: cbk_gels_batch_plan, dgels_batch, ...)
+
+class GelsProblem(ctypes.Structure):
+    """cbk_gels_problem: sizes, leading dimensions and element offsets of one
+    column-major problem of an uneven batch"""
+    _fields_ = [("m", ctypes.c_int), ("n", ctypes.c_int), ("nrhs", ctypes.c_int),
+                ("lda", ctypes.c_int), ("ldb", ctypes.c_int),
+                ("a_off", ctypes.c_longlong), ("b_off", ctypes.c_longlong)]
+
+
+BATCH_NCLASS = 6
+
+
+class Tuning(ctypes.Structure):
+    """cbk_tuning"""
+    _fields_ = [("narrow_max_cols", ctypes.c_int * 3),
+                ("blocks_per_cu", ctypes.c_int * BATCH_NCLASS)]
+
+
+def _bind_batch():
+    L = lib()
+    if getattr(L, "_batch_bound", False):
+        return L
+    P = ctypes.c_void_p
+    L.cbk_gels_batch_plan.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int,
+                                      ctypes.POINTER(GelsProblem), ctypes.POINTER(P),
+                                      ctypes.POINTER(ctypes.c_int)]
+    L.cbk_gels_batch_plan.restype = ctypes.c_int
+    L.cbk_batch_plan_worksize.argtypes = [P]
+    L.cbk_batch_plan_worksize.restype = ctypes.c_size_t
+    L.cbk_batch_plan_classes.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+    L.cbk_batch_plan_classes.restype = ctypes.c_int
+    for name in ("dgels_batch", "sgels_batch"):
+        f = getattr(L, name)
+        f.argtypes = [P, P, P, P, P, ctypes.c_size_t]
+        f.restype = ctypes.c_int
+    L.cbk_batch_plan_destroy.argtypes = [P]
+    L.cbk_batch_plan_destroy.restype = ctypes.c_int
+    L.cbk_autotune.argtypes = []
+    L.cbk_autotune.restype = ctypes.c_int
+    L.cbk_get_tuning.argtypes = [ctypes.POINTER(Tuning)]
+    L.cbk_get_tuning.restype = ctypes.c_int
+    L.cbk_set_tuning.argtypes = [ctypes.POINTER(Tuning)]
+    L.cbk_set_tuning.restype = ctypes.c_int
+    L._batch_bound = True
+    return L
+
+
+class GelsBatchPlan:
+    """A plan over an uneven batch (cbk_gels_batch_plan). problems: a sequence
+    of (m, n, nrhs, lda, ldb, a_off, b_off). Raises ValueError with the code
+    (and the index of a bad descriptor) on invalid input."""
+
+    def __init__(self, prec, trans, problems):
+        L = _bind_batch()
+        arr = (GelsProblem * max(1, len(problems)))()
+        for i, p in enumerate(problems):
+            arr[i] = GelsProblem(*[int(x) for x in p])
+        self._h = ctypes.c_void_p()
+        bad = ctypes.c_int(-1)
+        st = L.cbk_gels_batch_plan(prec.encode(), trans.encode(), len(problems), arr,
+                                   ctypes.byref(self._h), ctypes.byref(bad))
+        if st != 0:
+            raise ValueError(f"cbk_gels_batch_plan returned {st} (bad descriptor {bad.value})")
+        self.prec = prec
+        self.worksize = L.cbk_batch_plan_worksize(self._h)
+
+    def classes(self):
+        cnt = (ctypes.c_int * BATCH_NCLASS)()
+        _bind_batch().cbk_batch_plan_classes(self._h, cnt)
+        return list(cnt)
+
+    def run(self, ap, bp, info, work, lwork=None):
+        """dgels_batch / sgels_batch on device tensors (info and work may be
+        None); lwork defaults to work's size in bytes"""
+        f = _bind_batch().dgels_batch if self.prec == "D" else _bind_batch().sgels_batch
+        if lwork is None:
+            lwork = work.numel() * work.element_size() if hasattr(work, "numel") else 0
+        return f(self._h, _ptr(ap), _ptr(bp), _ptr(info), _ptr(work), lwork)
+
+    def close(self):
+        if self._h:
+            _bind_batch().cbk_batch_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune():
+    return _bind_batch().cbk_autotune()
+
+
+def get_tuning():
+    t = Tuning()
+    assert _bind_batch().cbk_get_tuning(ctypes.byref(t)) == 0
+    return {"narrow_max_cols": list(t.narrow_max_cols), "blocks_per_cu": list(t.blocks_per_cu)}
+
+
+def set_tuning(narrow_max_cols, blocks_per_cu):
+    t = Tuning()
+    t.narrow_max_cols[:] = list(narrow_max_cols)
+    t.blocks_per_cu[:] = list(blocks_per_cu)
+    return _bind_batch().cbk_set_tuning(ctypes.byref(t))

@@ -1,0 +1,287 @@
+/* batch_gels_impl.h -- least squares (gels, m >= n, trans 'N') over an uneven
+ * batch: problems of different m, n, nrhs, each stored column-major on its
+ * own, taken from a work queue by persistent grids.
+ *
+ * Column ownership. A problem is the m x (n + nrhs) matrix [A | B]; lane c of
+ * the problem's lane group owns column c in registers, f[i] = row i (rows >= m
+ * zero). Householder QR (the reference's geqr2 and larf,
+ * src/cbk_geqrf_compact.hpp:74-131, src/cbk_ormqr_compact.hpp:68-105) then
+ * runs per reflector kk:
+ *   tail   = sum_{i>kk} f_kk[i]^2            (lane kk, its own column)
+ *   larfg  : norm = sqrt(x0^2 + tail), beta = x0 >= 0 ? -norm : norm,
+ *            tau = (beta - x0)/beta, the body of column kk scaled by
+ *            1/(x0 - beta), R(kk,kk) = beta (tau = 0, column unchanged, when
+ *            tail = 0)
+ *   apply  : every column c > kk, B's included:
+ *            w = f_c[kk] + sum_{i>kk} v_i f_c[i], w *= tau,
+ *            f_c[kk] -= w, f_c[i] -= v_i w
+ * so Q^T B is formed in the B columns as the factorization proceeds (the
+ * reference's ormqr, forward). Back substitution R X = (Q^T B)(0:n) then runs
+ * in the B columns, column-oriented (x_l = b_l / R(l,l), b_i -= R(i,l) x_l).
+ * The reflector element v_i that every column needs lives in lane kk:
+ *   - team kernel (n + nrhs <= 16): 16 lanes per problem, 4 problems per
+ *     wavefront, v_i reaches the team by a DPP row broadcast fused into the
+ *     fma (reflectors and pivots unrolled at compile time);
+ *   - wavefront kernel (n + nrhs <= 64): one problem per wavefront, v_i read
+ *     with v_readlane into a scalar register; the reflector loop runs at run
+ *     time and row indices relative to it are wave-uniform branches.
+ * The team kernel sums in row order as the reference does; the wavefront
+ * kernel keeps four partial sums (rows i mod 4). Results agree with the
+ * reference to rounding, which the tests bound.
+ *
+ * Work distribution: one 32-bit counter per class, cleared before the launch;
+ * a wavefront takes the next 4 problems (team kernel) or 1 (wavefront kernel)
+ * with one atomic add and exits when the queue is empty. The plan orders each
+ * class longest first, so the tail of the queue is short problems.
+ */
+#ifndef CBK_BATCH_GELS_IMPL_H
+#define CBK_BATCH_GELS_IMPL_H
+
+#include "cbk_internal.h"
+#include "team_common.h"
+
+namespace cbk {
+namespace batch {
+
+using namespace team;
+
+template <class T> struct Larfg {
+    T tau, inv, rdiag;
+};
+
+/* larfg_pack (src/cbk_geqrf_compact.hpp:74-90): no rescaling, direct sqrt */
+template <class T> __device__ __forceinline__ Larfg<T> larfg(T x0, T tail)
+{
+    Larfg<T> h;
+    const T norm = sqrtT(fmaT(x0, x0, tail));
+    const T beta = (x0 >= T(0)) ? -norm : norm;
+    const bool has = tail > T(0);
+    h.tau = has ? (beta - x0) / beta : T(0);
+    h.inv = has ? T(1) / (x0 - beta) : T(0);
+    h.rdiag = has ? beta : x0;
+    return h;
+}
+
+/* ---- team kernel: n + nrhs <= 16, m <= MMAX --------------------------------- */
+template <class T, int MMAX>
+__global__ __launch_bounds__(256) void gels_team_kernel(BatchArgs<T> p)
+{
+    const int c = threadIdx.x & (TS - 1);
+    const int sub = (threadIdx.x >> 4) & 3; /* team within the wavefront */
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(p.counter, 4u);
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (base >= (unsigned)p.count) return; /* the whole wavefront */
+        const int k = (int)base + sub;
+        const bool live = k < p.count;
+        const BatchProblem d = p.desc[live ? k : (int)base];
+        const int m = d.m, n = live ? d.n : 0, w = live ? d.n + d.nrhs : 0;
+        const bool isA = c < n, isB = c >= n && c < w;
+        const bool has = isA || isB;
+        T *col = isA ? p.a + d.a_off + (long long)c * d.lda
+                     : p.b + d.b_off + (long long)(isB ? c - n : 0) * d.ldb;
+
+        T f[MMAX];
+        sfor<0, MMAX>([&](auto i) { f[i] = T(0); });
+        if (has) {
+            sfor<0, MMAX>([&](auto i) {
+                if (i < m) f[i] = col[i];
+            });
+        }
+
+        sfor<0, TS>([&](auto kc) {
+            constexpr int kk = kc;
+            if (kk < n) {
+                T tl = T(0);
+                sfor<kk + 1, MMAX>([&](auto i) { tl = fmaT(f[i], f[i], tl); });
+                const Larfg<T> h = larfg(bcast<kk>(f[kk]), bcast<kk>(tl));
+                if (c == kk) {
+                    sfor<kk + 1, MMAX>([&](auto i) { f[i] = f[i] * h.inv; });
+                    f[kk] = h.rdiag;
+                }
+                T wv = f[kk];
+                sfor<kk + 1, MMAX>([&](auto i) { fma_bc<kk>(wv, f[i], f[i]); });
+                wv = (c > kk) ? wv * h.tau : T(0);
+                f[kk] = f[kk] - wv;
+                sfor<kk + 1, MMAX>([&](auto i) { fnma_bc<kk>(f[i], f[i], wv); });
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
+
+        /* info: the first exact zero on R's diagonal (LAPACK dtrtrs's test) */
+        int bad = 0;
+        sfor<0, TS>([&](auto lc) {
+            constexpr int l = lc;
+            const T dl = bcast<l>(f[l]);
+            if (l < n && bad == 0 && dl == T(0)) bad = l + 1;
+        });
+        if (live && c == 0 && p.info) p.info[d.idx] = bad;
+        if (bad == 0) {
+            sfor_down<0, TS>([&](auto lc) {
+                constexpr int l = lc;
+                if (l < n) {
+                    const T dl = bcast<l>(f[l]);
+                    const T x = isB ? f[l] / dl : T(0);
+                    f[l] = isB ? x : f[l];
+                    sfor<0, l>([&](auto i) { fnma_bc<l>(f[i], f[i], x); });
+                }
+            });
+        }
+        if (has) {
+            sfor<0, MMAX>([&](auto i) {
+                if (i < m) col[i] = f[i];
+            });
+        }
+    }
+}
+
+/* ---- wavefront kernel: n + nrhs <= 64, m <= MMAX ----------------------------
+ * The reflector loop runs at run time, so each lane keeps a window of its
+ * column that starts at the current reflector's row: f[i] = row kk + i. Row kk
+ * is final once reflector kk is applied; it is stored and the window shifts
+ * up by one (zeros enter at the bottom, where rows >= m already are zero). The
+ * back substitution walks a reversed window up from row n-1 the same way.
+ * Every index into f is a compile-time constant, and rows past the end are
+ * zeros that every update leaves zero. */
+/* x, as a value the compiler must take to depend on dep. A v_readlane of x
+ * then cannot be merged with an earlier one or hoisted above dep; chaining
+ * dep to the result four rows back keeps about four broadcast values (eight
+ * scalar registers) alive instead of all 2*MMAX. */
+template <class T> __device__ __forceinline__ T after(T x, T dep)
+{
+    asm volatile("" : "+v"(x) : "v"(dep));
+    return x;
+}
+
+template <class T, int MMAX>
+__global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
+{
+    const int c = threadIdx.x & 63;
+    for (;;) {
+        unsigned kq = 0;
+        if (c == 0) kq = atomicAdd(p.counter, 1u);
+        kq = __builtin_amdgcn_readfirstlane(kq);
+        if (kq >= (unsigned)p.count) return;
+        const BatchProblem d = p.desc[kq];
+        const int m = d.m, n = d.n, w = d.n + d.nrhs;
+        const bool isA = c < n, isB = c >= n && c < w;
+        const bool has = isA || isB;
+        T *col = isA ? p.a + d.a_off + (long long)c * d.lda
+                     : p.b + d.b_off + (long long)(isB ? c - n : 0) * d.ldb;
+
+        T f[MMAX];
+        sfor<0, MMAX>([&](auto i) { f[i] = T(0); });
+        if (has) {
+            sfor<0, MMAX>([&](auto i) {
+                if (i < m) f[i] = col[i];
+            });
+        }
+
+        T own = T(0); /* R(c,c) of an A lane */
+        for (int kk = 0; kk < n; ++kk) {
+            T tl = T(0);
+            sfor<1, MMAX>([&](auto i) { tl = fmaT(f[i], f[i], tl); });
+            const Larfg<T> h = larfg(rdlane(f[0], kk), rdlane(tl, kk));
+            if (c == kk) {
+                sfor<1, MMAX>([&](auto i) { f[i] = f[i] * h.inv; });
+                f[0] = h.rdiag;
+            }
+            /* w = f[0] + sum v_i f[i], four partial sums */
+            T acc[4] = {f[0], T(0), T(0), T(0)};
+            sfor<1, MMAX>([&](auto i) {
+                const T fi = (i > 4) ? after(f[i], acc[i % 4]) : f[i];
+                acc[i % 4] = fmaT(rdlane(fi, kk), fi, acc[i % 4]);
+            });
+            T wv = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            wv = (c > kk) ? wv * h.tau : T(0);
+            f[0] = f[0] - wv;
+            sfor<1, MMAX>([&](auto i) {
+                const T fi = after(f[i], (i > 4) ? f[i - 4] : wv);
+                f[i] = fmaT(-rdlane(fi, kk), wv, fi);
+            });
+            own = (c == kk) ? f[0] : own;
+            if (has) col[kk] = f[0];
+            sfor<0, MMAX - 1>([&](auto i) { f[i] = f[i + 1]; });
+            f[MMAX - 1] = T(0);
+        }
+        if (has) {
+            /* rows n..m-1: Q^T B below the solution, reflector bodies of A */
+            sfor<0, MMAX>([&](auto i) {
+                if (i < m - n) col[n + i] = f[i];
+            });
+        }
+
+        /* info: the first exact zero on R's diagonal (LAPACK dtrtrs's test) */
+        int bad = 0;
+        for (int l = 0; l < n; ++l) {
+            if (rdlane(own, l) == T(0)) {
+                bad = l + 1;
+                break;
+            }
+        }
+        if (c == 0 && p.info) p.info[d.idx] = bad;
+        if (bad != 0) continue;
+
+        /* R X = (Q^T B)(0:n): g[j] = row l - j of the lane's column at step l */
+        T g[MMAX];
+        sfor<0, MMAX>([&](auto j) { g[j] = T(0); });
+        if (has) {
+            sfor<0, MMAX>([&](auto j) {
+                if (j < n) g[j] = col[n - 1 - j];
+            });
+        }
+        for (int l = n - 1; l >= 0; --l) {
+            const T dl = rdlane(own, l);
+            const T x = isB ? g[0] / dl : T(0);
+            if (isB) col[l] = x;
+            sfor<1, MMAX>([&](auto j) {
+                const T gj = after(g[j], (j > 4) ? g[j - 4] : x);
+                g[j] = fmaT(-rdlane(gj, l), x, gj);
+            });
+            sfor<0, MMAX - 1>([&](auto j) { g[j] = g[j + 1]; });
+            g[MMAX - 1] = T(0);
+        }
+    }
+}
+
+template <class T> hipError_t launch_class(int cls, const BatchArgs<T> &args, unsigned blocks,
+                                           hipStream_t s)
+{
+    const dim3 grid(blocks), block(256);
+    switch (cls) {
+    case 0: hipLaunchKernelGGL((gels_team_kernel<T, 16>), grid, block, 0, s, args); break;
+    case 1: hipLaunchKernelGGL((gels_team_kernel<T, 32>), grid, block, 0, s, args); break;
+    case 2: hipLaunchKernelGGL((gels_team_kernel<T, 64>), grid, block, 0, s, args); break;
+    case 3: hipLaunchKernelGGL((gels_wave_kernel<T, 32>), grid, block, 0, s, args); break;
+    case 4: hipLaunchKernelGGL((gels_wave_kernel<T, 64>), grid, block, 0, s, args); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+} /* namespace batch */
+
+template <class T>
+hipError_t launch_gels_batch(int cls, const BatchArgs<T> &args, unsigned blocks, hipStream_t s)
+{
+    return batch::launch_class<T>(cls, args, blocks, s);
+}
+
+template <class T> hipError_t gels_batch_occupancy(int cls, int *blocks)
+{
+    using namespace batch;
+    switch (cls) {
+    case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_team_kernel<T, 16>, 256, 0);
+    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_team_kernel<T, 32>, 256, 0);
+    case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_team_kernel<T, 64>, 256, 0);
+    case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_kernel<T, 32>, 256, 0);
+    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_kernel<T, 64>, 256, 0);
+    default: *blocks = 0; return hipErrorInvalidValue;
+    }
+}
+
+} /* namespace cbk */
+
+#endif /* CBK_BATCH_GELS_IMPL_H */

@@ -25,6 +25,13 @@ thread_local hipError_t t_last_error = hipSuccess;
 hipStream_t current_stream() { return t_stream; }
 int current_policy() { return t_policy; }
 
+int launch_status(hipError_t e)
+{
+    if (e == hipSuccess) return 0;
+    t_last_error = e;
+    return CBK_ERR_LAUNCH;
+}
+
 namespace {
 
 /* src/cbk.cpp:46-59 */
@@ -35,12 +42,7 @@ inline bool vlen_ok(int V)
     return V == 2 || V == 4 || V == 8 || V == 16 || V == 32 || V == 64;
 }
 
-inline int status(hipError_t e)
-{
-    if (e == hipSuccess) return 0;
-    t_last_error = e;
-    return CBK_ERR_LAUNCH;
-}
+inline int status(hipError_t e) { return launch_status(e); }
 
 /* group_stride (src/cbk_common.hpp:210-214) */
 inline size_t gstride(bool row, int ld, int rows, int cols, int V)

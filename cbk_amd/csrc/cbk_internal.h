@@ -15,6 +15,10 @@ hipStream_t current_stream();
 /* the kernel policy of the calling host thread (cbk_set_policy) */
 int current_policy();
 
+/* 0 for hipSuccess; otherwise records e for cbk_last_error_string and
+ * returns CBK_ERR_LAUNCH */
+int launch_status(hipError_t e);
+
 /* One operand of a compact batch: base pointer, leading dimension, layout and
  * scalars per group (group_stride of the reference, src/cbk_common.hpp:210). */
 template <class T> struct Operand {
@@ -84,6 +88,41 @@ bool qr_team_supports(int p, int q, int nrhs, bool fp64);
 template <class T>
 hipError_t launch_qr_team(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
                           Operand<T> B, int V, int nm, int *info);
+
+/* ---- uneven batches (batch_gels_impl.h, batch_plan.cpp) ---- */
+
+/* one problem of an uneven batch as the kernels read it (device table) */
+struct BatchProblem {
+    int m, n, nrhs, lda, ldb;
+    int idx;            /* index in the caller's batch: info[idx] */
+    long long a_off, b_off;
+    long long tau_off;  /* general kernel: its tau in the workspace */
+};
+
+/* the kernels of one precision for one size class: one persistent grid over
+ * `count` problems starting at `desc`, taking work from `counter` (zero at
+ * launch) */
+template <class T> struct BatchArgs {
+    T *a, *b;
+    int *info;
+    const BatchProblem *desc;
+    unsigned *counter;
+    int count;
+    T *tau;   /* general kernel only */
+    bool tran;
+};
+
+/* classes: 0-2 team kernel with 16/32/64 row registers, 3-4 wavefront kernel
+ * with 32/64, 5 the general lane kernel */
+constexpr int kBatchClasses = 6;
+constexpr int kBatchGeneral = 5;
+template <class T>
+hipError_t launch_gels_batch(int cls, const BatchArgs<T> &args, unsigned blocks, hipStream_t s);
+template <class T>
+hipError_t launch_gels_batch_lane(const BatchArgs<T> &args, hipStream_t s);
+/* resident 256-thread blocks per compute unit of class cls's kernel */
+template <class T> hipError_t gels_batch_occupancy(int cls, int *blocks);
+hipError_t launch_clear_u32(unsigned *p, int n, hipStream_t s);
 
 /* B(0:rows, 0:cols) := 0 in every group */
 template <class T>

@@ -392,8 +392,39 @@ __global__ __launch_bounds__(kBlock) void orgqr_lane_kernel(int m, int n, int k,
     }
 }
 
-/* gels_compact_group (src/cbk_gels_compact.hpp:84-106, 113-146); with info, a
- * zero R(i,i) stops the lane where LAPACK dgels's dtrtrs would */
+/* gels_compact_group (src/cbk_gels_compact.hpp:84-106, 113-146) on one lane's
+ * views; with info, a zero R(i,i) stops the lane where LAPACK dgels's dtrtrs
+ * would. Returns that info code. */
+template <class T>
+__device__ int gels_one(bool tran, int m, int n, int nrhs, const LaneView<T> &Av,
+                        const LaneView<T> &Bv, T *tau, int V, bool with_info)
+{
+    const bool tall = (m >= n);
+    const int p = tall ? m : n, q = tall ? n : m;
+    const bool overdet = (tall != tran);
+    const LaneView<T> F = tall ? Av : Av.tr();
+    geqrf_lane<T>(p, q, F, tau, V);
+    int bad = 0;
+    if (with_info) {
+        for (int i = 0; i < q && !bad; ++i)
+            if (F(i, i) == T(0)) bad = i + 1;
+    }
+    if (overdet) {
+        ormqr_lane<T>(true, p, nrhs, q, F, tau, Bv, V);
+        if (bad) return bad;
+        trsm_group<T>(true, true, false, false, q, nrhs, T(1), F, Bv);
+    }
+    else {
+        if (bad) return bad;
+        trsm_group<T>(true, true, true, false, q, nrhs, T(1), F, Bv);
+        for (int j = 0; j < nrhs; ++j)
+            for (int i = q; i < p; ++i)
+                Bv(i, j) = T(0);
+        ormqr_lane<T>(false, p, nrhs, q, F, tau, Bv, V);
+    }
+    return bad;
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void gels_lane_kernel(bool tran, int m, int n, int nrhs,
                                                            Operand<T> A, Operand<T> B, T *tau_,
@@ -404,33 +435,38 @@ __global__ __launch_bounds__(kBlock) void gels_lane_kernel(bool tran, int m, int
     if (idx >= nlanes) return;
     const long g = idx / V;
     const int v = (int)(idx % V);
-    const bool tall = (m >= n);
-    const int p = tall ? m : n, q = tall ? n : m;
-    const bool overdet = (tall != tran);
-    const LaneView<T> Av = lane_view(A, g, v, V);
-    const LaneView<T> F = tall ? Av : Av.tr();
-    const LaneView<T> Bv = lane_view(B, g, v, V);
-    T *tau = tau_ + (size_t)g * q * V + v;
-    geqrf_lane<T>(p, q, F, tau, V);
+    const int q = m < n ? m : n;
+    const bool real = info && idx < nm;
+    const int bad = gels_one<T>(tran, m, n, nrhs, lane_view(A, g, v, V), lane_view(B, g, v, V),
+                                tau_ + (size_t)g * q * V + v, V, real);
+    if (real) info[idx] = bad;
+}
+
+/* one problem of an uneven batch per lane, column-major with V = 1: the
+ * general case of gels_batch (transposed or underdetermined problems, sizes
+ * the register kernels do not hold, empty problems) */
+template <class T>
+__global__ __launch_bounds__(kBlock) void gels_batch_lane_kernel(BatchArgs<T> p)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.count) return;
+    const BatchProblem d = p.desc[k];
+    const LaneView<T> Av{p.a + d.a_off, 1, d.lda, 1};
+    const LaneView<T> Bv{p.b + d.b_off, 1, d.ldb, 1};
+    const int mx = d.m > d.n ? d.m : d.n, q = d.m < d.n ? d.m : d.n;
     int bad = 0;
-    if (info && idx < nm) {
-        for (int i = 0; i < q && !bad; ++i)
-            if (F(i, i) == T(0)) bad = i + 1;
-        info[idx] = bad;
+    if (d.nrhs > 0 && mx > 0) {
+        if (q == 0) {
+            /* gels_compact with q = 0: B := 0 (src/cbk_gels_compact.hpp:127-131) */
+            for (int j = 0; j < d.nrhs; ++j)
+                for (int i = 0; i < mx; ++i)
+                    Bv(i, j) = T(0);
+        }
+        else {
+            bad = gels_one<T>(p.tran, d.m, d.n, d.nrhs, Av, Bv, p.tau + d.tau_off, 1, p.info != nullptr);
+        }
     }
-    if (overdet) {
-        ormqr_lane<T>(true, p, nrhs, q, F, tau, Bv, V);
-        if (bad) return;
-        trsm_group<T>(true, true, false, false, q, nrhs, T(1), F, Bv);
-    }
-    else {
-        if (bad) return;
-        trsm_group<T>(true, true, true, false, q, nrhs, T(1), F, Bv);
-        for (int j = 0; j < nrhs; ++j)
-            for (int i = q; i < p; ++i)
-                Bv(i, j) = T(0);
-        ormqr_lane<T>(false, p, nrhs, q, F, tau, Bv, V);
-    }
+    if (p.info) p.info[d.idx] = bad;
 }
 
 /* ------------------------------------------------------------------ */
@@ -620,6 +656,26 @@ hipError_t launch_orgqr_lane(int m, int n, int k, Operand<T> A, const T *tau, in
     return hipGetLastError();
 }
 
+template <class T> hipError_t launch_gels_batch_lane(const BatchArgs<T> &args, hipStream_t s)
+{
+    if (args.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL((gels_batch_lane_kernel<T>), dim3(nblocks(args.count)), dim3(kBlock), 0, s,
+                       args);
+    return hipGetLastError();
+}
+
+__global__ void clear_u32_kernel(unsigned *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_clear_u32(unsigned *p, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(clear_u32_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p, n);
+    return hipGetLastError();
+}
+
 template <class T>
 hipError_t launch_gels_lane(bool tran, int m, int n, int nrhs, Operand<T> A, Operand<T> B,
                             T *tau, int V, int nm, int *info)
@@ -652,6 +708,7 @@ hipError_t launch_mixed_posv_lane(bool up, int n, int nrhs, Operand<double> A,
                                              const T *, Operand<T>, int, int);            \
     template hipError_t launch_orgqr_lane<T>(int, int, int, Operand<T>, const T *, int,   \
                                              int);                                        \
+    template hipError_t launch_gels_batch_lane<T>(const BatchArgs<T> &, hipStream_t);    \
     template hipError_t launch_gels_lane<T>(bool, int, int, int, Operand<T>, Operand<T>,  \
                                             T *, int, int, int *);
 

@@ -70,6 +70,33 @@ template <int S> __device__ __forceinline__ void fnma_bc(float &acc, float x, fl
         : "v"(x), "v"(a), "i"(S));
 }
 
+/* acc := fma(x_S, a, acc), the broadcast riding on the fma as above */
+template <int S> __device__ __forceinline__ void fma_bc(double &acc, double x, double a)
+{
+    asm("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(x), "v"(a), "i"(S));
+}
+template <int S> __device__ __forceinline__ void fma_bc(float &acc, float x, float a)
+{
+    asm("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(x), "v"(a), "i"(S));
+}
+
+/* the value of x in lane l of the wavefront (l wave-uniform) */
+__device__ __forceinline__ double rdlane(double x, int l)
+{
+    int2 w = __builtin_bit_cast(int2, x);
+    w.x = __builtin_amdgcn_readlane(w.x, l);
+    w.y = __builtin_amdgcn_readlane(w.y, l);
+    return __builtin_bit_cast(double, w);
+}
+__device__ __forceinline__ float rdlane(float x, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+
 /* Index of the calling wavefront in its block, as a value the compiler knows
  * to be wave-uniform. Bases derived from it (group pointers, buffer
  * resources) stay in SGPRs; derived from threadIdx.x directly they would be

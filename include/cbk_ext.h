@@ -95,6 +95,79 @@ int dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap
 /* bytes of workspace dsposv_compact needs for this call shape (0 if none) */
 size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int V, int nm);
 
+/* ---- uneven batches -------------------------------------------------------
+ * No reference counterpart. A batch of independent least-squares problems
+ * min || B_k - op(A_k) X_k || of different sizes, each stored column-major on
+ * its own (not compact): problem k's A is m x n at ap + a_off (lda >=
+ * max(1, m)), its B is max(m, n) x nrhs at bp + b_off (ldb >= max(1, m, n)).
+ * Per problem the result is that of gels_compact on that one matrix: A
+ * receives its QR (LQ) factorization, the first n (m) rows of B the solution,
+ * for a least-squares problem the remaining rows Q^T B; info[k] = 0, or the
+ * 1-based index of the first exact zero on R's diagonal (LAPACK dgels), B
+ * then left as Q^T B. Offsets are in elements.
+ *
+ * Usage: plan once per batch shape on the host, query the workspace, then
+ * call dgels_batch / sgels_batch as often as wanted. The plan sorts the
+ * problems into size classes, orders each class longest first, picks each
+ * class's kernel from the tuning table (cbk_autotune), and records all of
+ * it as one hipGraph: a counter-clearing kernel followed by one persistent
+ * work-queue kernel per class, the classes running concurrently. The graph
+ * is launched on the caller's stream; it is re-recorded only when the
+ * pointers passed to the call change. The first call with a given work
+ * buffer copies the plan's problem table into it; keep work untouched
+ * between calls. A plan is not thread-safe: one host thread at a time. */
+typedef struct cbk_gels_problem {
+    int m, n, nrhs, lda, ldb;
+    long long a_off, b_off;
+} cbk_gels_problem;
+
+typedef struct cbk_batch_plan *cbk_batch_plan_t;
+
+/* prec 'D' (dgels_batch) or 'S' (sgels_batch); trans 'N', 'T' or 'C' as
+ * gels_compact. Codes: -1 prec, -2 trans, -3 nprob < 0, -4 desc NULL,
+ * -5 a descriptor is invalid (its index goes to *bad if bad is not NULL),
+ * -6 plan NULL. On success *plan is a new plan (host memory only). */
+int cbk_gels_batch_plan(char prec, char trans, int nprob, const cbk_gels_problem *desc,
+                        cbk_batch_plan_t *plan, int *bad);
+
+/* device bytes of workspace the plan's calls need */
+size_t cbk_batch_plan_worksize(cbk_batch_plan_t plan);
+
+/* problems per kernel class (count[CBK_BATCH_NCLASS]); returns the number
+ * of classes. Classes: 0-2 one problem per 16-lane team (rows <= 16, 32,
+ * 64), 3-4 one problem per wavefront (rows <= 32, 64), 5 the general
+ * one-lane-per-problem kernel (transposed or underdetermined problems, more
+ * than 64 rows or n + nrhs > 64 columns, empty problems). */
+#define CBK_BATCH_NCLASS 6
+int cbk_batch_plan_classes(cbk_batch_plan_t plan, int *count);
+
+/* Codes: -1 plan NULL or of the other precision, -2 ap, -3 bp, -5 work NULL
+ * with a nonzero worksize, -6 lwork too small; CBK_ERR_LAUNCH on a HIP error. */
+int dgels_batch(cbk_batch_plan_t plan, double *ap, double *bp, int *info, void *work,
+                size_t lwork);
+int sgels_batch(cbk_batch_plan_t plan, float *ap, float *bp, int *info, void *work,
+                size_t lwork);
+
+int cbk_batch_plan_destroy(cbk_batch_plan_t plan);
+
+/* ---- tuning ----------------------------------------------------------------
+ * The batch plans choose, per row bucket (rows <= 16, <= 32, <= 64), between
+ * the team kernel (one problem per 16 lanes, n + nrhs <= 16 columns) and the
+ * wavefront kernel (one problem per 64 lanes) by the largest column count
+ * narrow_max_cols[b] the team kernel should take, and size each persistent
+ * grid to blocks_per_cu[c] blocks of 256 threads per compute unit.
+ * cbk_autotune measures both on the current device and stream with synthetic
+ * problems (allocating and freeing its own device memory) and installs the
+ * result for plans created afterwards on this device. */
+typedef struct cbk_tuning {
+    int narrow_max_cols[3];
+    int blocks_per_cu[CBK_BATCH_NCLASS];
+} cbk_tuning;
+
+int cbk_autotune(void);
+int cbk_get_tuning(cbk_tuning *t);
+int cbk_set_tuning(const cbk_tuning *t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -843,6 +843,56 @@ ORACLE_DEFS(double, d)
 ORACLE_DEFS(float, s)
 #undef ORACLE_DEFS
 
+} /* extern "C" */
+
+namespace {
+/* the reference's gels on one column-major matrix: the compact view with one
+ * slot (V = 1), tau on the stack; the quick returns of gels<T> per problem */
+template <class T>
+int gels_batch(char trans, int nprob, const oracle_gels_problem *desc, T *ap, T *bp, int *info)
+{
+    if (!opt(trans, 'N') && !opt(trans, 'T') && !opt(trans, 'C')) return -1;
+    if (nprob < 0) return -2;
+    const bool tran = !opt(trans, 'N');
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int k = 0; k < nprob; ++k) {
+        const oracle_gels_problem &d = desc[k];
+        const long mx = d.m > d.n ? d.m : d.n;
+        const bool tall = d.m >= d.n;
+        const long p = tall ? d.m : d.n, q = tall ? d.n : d.m;
+        const LV<T> A = make_view(ap + d.a_off, false, d.lda, 1);
+        const LV<T> B = make_view(bp + d.b_off, false, d.ldb, 1);
+        int bad = 0;
+        if (d.nrhs > 0 && mx > 0) {
+            if (q == 0) {
+                zero_block(B, 0, p, d.nrhs);
+            }
+            else {
+                std::vector<T> tau(q);
+                gels_lane<T>(tall != tran, p, q, d.nrhs, tall ? A : A.tr(), tau.data(), B, 1,
+                             info ? &bad : nullptr);
+            }
+        }
+        if (info) info[k] = bad;
+    }
+    return 0;
+}
+} /* namespace */
+
+extern "C" {
+
+int oracle_dgels_batch(char trans, int nprob, const oracle_gels_problem *desc, double *ap,
+                       double *bp, int *info)
+{
+    return gels_batch<double>(trans, nprob, desc, ap, bp, info);
+}
+
+int oracle_sgels_batch(char trans, int nprob, const oracle_gels_problem *desc, float *ap,
+                       float *bp, int *info)
+{
+    return gels_batch<float>(trans, nprob, desc, ap, bp, info);
+}
+
 /* LAPACK 3.12 dsposv.f, per matrix, on the compact layout. The inner single
  * precision factor and solves are this file's restatement of the reference's
  * spotrf/spotrs lane kernels. Per-matrix codes follow dsposv's ITER: 0..30 the

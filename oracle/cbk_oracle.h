@@ -82,6 +82,21 @@ int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, i
                           const double *bp, int ldbp, double *xp, int ldxp, void *work,
                           size_t lwork, int *iter, int *info, int V, int nm);
 
+/* One uneven batch (no reference counterpart): problem k is gels_compact on
+ * the single column-major matrix A_k (m x n, lda) at ap + a_off and B_k
+ * (max(m,n) x nrhs, ldb) at bp + b_off, i.e. the reference's
+ * gels_compact_group (src/cbk_gels_compact.hpp:84-146) with a one-matrix
+ * view; info[k] as the _info variants. Same struct layout as the library's
+ * cbk_gels_problem. */
+typedef struct oracle_gels_problem {
+    int m, n, nrhs, lda, ldb;
+    long long a_off, b_off;
+} oracle_gels_problem;
+int oracle_dgels_batch(char trans, int nprob, const oracle_gels_problem *desc, double *ap,
+                       double *bp, int *info);
+int oracle_sgels_batch(char trans, int nprob, const oracle_gels_problem *desc, float *ap,
+                       float *bp, int *info);
+
 #ifdef __cplusplus
 }
 #endif
