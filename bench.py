@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nm", type=int, default=NM)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch gels leg")
+    ap.add_argument("--uneven-nprob", type=int, default=K5_NPROB)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (the known-size kernel "
                          "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -154,6 +156,167 @@ def cpu_baseline(A, B):
     return {"value": nm / best, "unit": "matrices/s", "cores": cores, "kind": kind,
             "sample": f"dposv_compact n=32 nrhs=4 FP64 V=8, {nm} matrices, best of 3, "
                       f"{os.path.basename(path) if path else 'oracle port'}; max backward error {eta:.1e}"}
+
+
+K5_NPROB = 500000
+PER_SIZE_SAMPLE = 256
+
+
+def uneven_inputs(nprob, seed=2027):
+    """the uneven least-squares batch: m ~ U[4,64], n ~ U[2,m], nrhs ~ U[1,8],
+    A and B entries ~ U(-1,1); each problem column-major with lda = ldb = m,
+    packed back to back. Returns (descriptor array, A pool, B pool) on the host."""
+    import numpy as np
+    import cbk_amd
+    r = np.random.default_rng(seed)
+    m = r.integers(4, 65, nprob)
+    n = 2 + (r.random(nprob) * (m - 1)).astype(np.int64)
+    nrhs = r.integers(1, 9, nprob)
+    asz, bsz = m * n, m * nrhs
+    desc = np.zeros(nprob, cbk_amd.problem_dtype())
+    desc["m"], desc["n"], desc["nrhs"], desc["lda"], desc["ldb"] = m, n, nrhs, m, m
+    desc["a_off"] = np.cumsum(asz) - asz
+    desc["b_off"] = np.cumsum(bsz) - bsz
+    A = r.uniform(-1, 1, int(asz.sum()))
+    B = r.uniform(-1, 1, int(bsz.sum()))
+    return desc, A, B
+
+
+def uneven_bench(torch, K, W, nprob=K5_NPROB):
+    """dgels_batch / sgels_batch over the uneven batch (one graph launch per
+    step, inputs re-copied outside the timed region), against the same problems
+    solved by dgels_compact once per distinct (m, n, nrhs), each size packed
+    compact with V = 64, as a caller without a batch API would"""
+    import numpy as np
+    import cbk_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    log(f"generating the uneven batch ({nprob} problems) ...")
+    desc, Ah, Bh = uneven_inputs(nprob)
+    m, n, q = desc["m"], desc["n"], desc["nrhs"]
+    nbytes = {"D": 8, "S": 4}
+    out = {"workload": f"gels, {nprob} problems, m~U[4,64], n~U[2,m], nrhs~U[1,8], U(-1,1), "
+                       "column-major per problem",
+           "nprob": nprob}
+    for prec, dt in (("D", np.float64), ("S", np.float32)):
+        A0 = torch.from_numpy(Ah.astype(dt)).cuda()
+        B0 = torch.from_numpy(Bh.astype(dt)).cuda()
+        A, B = A0.clone(), B0.clone()
+        plan = cbk_amd.GelsBatchPlan(prec, "N", desc)
+        work = torch.empty(max(plan.worksize, 1), dtype=torch.uint8, device="cuda")
+        info = torch.empty(nprob, dtype=torch.int32, device="cuda")
+        ts = []
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        stream = torch.cuda.current_stream()
+        for s in range(W + K):
+            A.copy_(A0)
+            B.copy_(B0)
+            e0.record(stream)
+            assert plan.run(A, B, info, work) == 0, cbk_amd.last_error()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if s >= W:
+                ts.append(e0.elapsed_time(e1) / 1e3)
+        t = sum(ts) / len(ts)
+        traffic = 2 * nbytes[prec] * int((m * n + m * q).sum())
+        res = {"ms": t * 1e3, "problems_per_s": nprob / t, "hbm_gbs": traffic / t / 1e9,
+               "bytes": traffic, "classes": plan.classes(),
+               "nonzero_info": int((info != 0).sum().item())}
+        # a sample against the oracle's batch gels on the host
+        if prec == "D":
+            import batchcase
+            ns = 2000
+            idx = np.random.default_rng(5).choice(nprob, ns, replace=False)
+            Ag, Bg = A.cpu().numpy(), B.cpu().numpy()
+            worst = 0.0
+            sd, sa, sb, As, Bs = [], [], [], [], []
+            for k in idx:
+                mk, nk, qk = int(m[k]), int(n[k]), int(q[k])
+                a0, b0 = int(desc["a_off"][k]), int(desc["b_off"][k])
+                As.append(Ah[a0:a0 + mk * nk].reshape(nk, mk).T.copy())
+                Bs.append(Bh[b0:b0 + mk * qk].reshape(qk, mk).T.copy())
+            sdesc, sap, sbp, _, _ = batchcase.make(np.random.default_rng(0),
+                                                   [(int(m[k]), int(n[k]), int(q[k])) for k in idx], pad=(0,))
+            for j, k in enumerate(idx):
+                mk, nk, qk, lda, ldb, a0, b0 = sdesc[j]
+                sap[a0:a0 + mk * nk] = As[j].T.reshape(-1)
+                sbp[b0:b0 + mk * qk] = Bs[j].T.reshape(-1)
+            assert batchcase.oracle_batch("d", "N", sdesc, sap, sbp) == 0
+            for j, k in enumerate(idx):
+                mk, nk, qk = int(m[k]), int(n[k]), int(q[k])
+                b0 = int(desc["b_off"][k])
+                Xg = Bg[b0:b0 + mk * qk].reshape(qk, mk).T[:nk]
+                Xo = batchcase.get_b(sbp, sdesc[j])[:nk]
+                worst = max(worst, float(np.abs(Xg - Xo).max() / max(np.abs(Xo).max(), 1e-300)))
+            res["sample_vs_oracle"] = {"problems": ns, "max_rel_diff_x": worst}
+        out["fp64" if prec == "D" else "fp32"] = res
+        log(f"uneven gels {prec}: {t * 1e3:.2f} ms/step, {nprob / t:.3e} problems/s, "
+            f"{traffic / t / 1e9:.0f} GB/s, classes {plan.classes()}")
+        del A0, B0, A, B, work, plan
+        torch.cuda.empty_cache()
+
+    # per-size compact launches, FP64
+    log("per-size compact launches ...")
+    key = (m * 100 + n) * 10 + q
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    bounds = np.flatnonzero(np.diff(ks)) + 1
+    starts = np.concatenate([[0], bounds])
+    ends = np.concatenate([bounds, [nprob]])
+    V = 64
+    ntotal = len(starts)
+    # a random sample of the sizes keeps this leg to seconds; the rate is the
+    # sampled problems over the sampled launches' time
+    pick = np.sort(np.random.default_rng(11).choice(ntotal, min(ntotal, PER_SIZE_SAMPLE), replace=False))
+    groups, apos, bpos = [], 0, 0
+    for s0, s1 in zip(starts[pick], ends[pick]):
+        ids = order[s0:s1]
+        mk, nk, qk = int(m[ids[0]]), int(n[ids[0]]), int(q[ids[0]])
+        ng = (len(ids) + V - 1) // V
+        groups.append((mk, nk, qk, len(ids), ids, apos, bpos))
+        apos += ng * V * mk * nk
+        bpos += ng * V * mk * qk
+    Apk = np.zeros(apos)
+    Bpk = np.zeros(bpos)
+    tau_max = 0
+    for mk, nk, qk, cnt, ids, a0, b0 in groups:
+        ng = (cnt + V - 1) // V
+        ga = Ah[desc["a_off"][ids][:, None] + np.arange(mk * nk)]          # (cnt, n*m) col-major
+        gb = Bh[desc["b_off"][ids][:, None] + np.arange(mk * qk)]
+        fa = np.zeros((ng * V, mk * nk))
+        fa[:cnt] = ga
+        eye = np.zeros((mk, nk))
+        eye[np.arange(nk), np.arange(nk)] = 1.0
+        fa[cnt:] = eye.T.reshape(-1)
+        fb = np.zeros((ng * V, mk * qk))
+        fb[:cnt] = gb
+        Apk[a0:a0 + ng * V * mk * nk] = fa.reshape(ng, V, mk * nk).transpose(0, 2, 1).reshape(-1)
+        Bpk[b0:b0 + ng * V * mk * qk] = fb.reshape(ng, V, mk * qk).transpose(0, 2, 1).reshape(-1)
+        tau_max = max(tau_max, ng * V * nk)
+    dA = torch.from_numpy(Apk).cuda()
+    dB = torch.from_numpy(Bpk).cuda()
+    dT = torch.empty(tau_max, dtype=torch.float64, device="cuda")
+    fn_, _ = cbk_amd.fn("d", "gels")
+    base_a, base_b, base_t = dA.data_ptr(), dB.data_ptr(), dT.data_ptr()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    stream = torch.cuda.current_stream()
+    e0.record(stream)
+    for mk, nk, qk, cnt, ids, a0, b0 in groups:
+        st = fn_(b"C", b"N", mk, nk, qk, ctypes.c_void_p(base_a + 8 * a0), mk,
+                 ctypes.c_void_p(base_b + 8 * b0), mk, ctypes.c_void_p(base_t), V, cnt)
+        assert st == 0
+    e1.record(stream)
+    torch.cuda.synchronize()
+    tp = e0.elapsed_time(e1) / 1e3
+    nsamp = sum(g[3] for g in groups)
+    out["per_size_fp64"] = {"ms_sampled": tp * 1e3, "problems_per_s": nsamp / tp,
+                            "launches_total": ntotal, "launches_sampled": len(groups),
+                            "problems_sampled": nsamp}
+    out["batch_over_per_size_fp64"] = out["fp64"]["problems_per_s"] / out["per_size_fp64"]["problems_per_s"]
+    log(f"per-size dgels_compact: {len(groups)} of {ntotal} sizes, {nsamp} problems, {tp * 1e3:.1f} ms")
+    del dA, dB, dT
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -288,6 +451,8 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
+    if not args.no_uneven:
+        result["uneven_gels"] = uneven_bench(torch, K=max(2, min(K, 5)), W=1, nprob=args.uneven_nprob)
     if not args.no_cpu:
         log("CPU baseline (reference dposv_compact on the host) ...")
         result["cpu_baseline"] = cpu_baseline(A_host, B_host)

@@ -161,6 +161,14 @@ class GelsProblem(ctypes.Structure):
 BATCH_NCLASS = 6
 
 
+def problem_dtype():
+    """numpy dtype laid out as cbk_gels_problem, for building large batches
+    without a Python loop"""
+    import numpy as np
+    return np.dtype([("m", "<i4"), ("n", "<i4"), ("nrhs", "<i4"), ("lda", "<i4"),
+                     ("ldb", "<i4"), ("a_off", "<i8"), ("b_off", "<i8")], align=True)
+
+
 class Tuning(ctypes.Structure):
     """cbk_tuning"""
     _fields_ = [("narrow_max_cols", ctypes.c_int * 3),
@@ -203,9 +211,15 @@ class GelsBatchPlan:
 
     def __init__(self, prec, trans, problems):
         L = _bind_batch()
-        arr = (GelsProblem * max(1, len(problems)))()
-        for i, p in enumerate(problems):
-            arr[i] = GelsProblem(*[int(x) for x in p])
+        if hasattr(problems, "dtype"):  # numpy array of PROBLEM_DTYPE
+            import numpy as np
+            assert problems.dtype == problem_dtype() and problems.flags.c_contiguous
+            arr = problems.ctypes.data_as(ctypes.POINTER(GelsProblem))
+            self._keep = problems
+        else:
+            arr = (GelsProblem * max(1, len(problems)))()
+            for i, p in enumerate(problems):
+                arr[i] = GelsProblem(*[int(x) for x in p])
         self._h = ctypes.c_void_p()
         bad = ctypes.c_int(-1)
         st = L.cbk_gels_batch_plan(prec.encode(), trans.encode(), len(problems), arr,
