@@ -158,7 +158,7 @@ class GelsProblem(ctypes.Structure):
                 ("a_off", ctypes.c_longlong), ("b_off", ctypes.c_longlong)]
 
 
-BATCH_NCLASS = 6
+BATCH_NCLASS = 7
 
 
 def problem_dtype():

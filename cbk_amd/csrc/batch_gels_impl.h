@@ -24,7 +24,9 @@
  *     fma (reflectors and pivots unrolled at compile time);
  *   - wavefront kernel (n + nrhs <= 64): one problem per wavefront, v_i read
  *     with v_readlane into a scalar register; the reflector loop runs at run
- *     time and row indices relative to it are wave-uniform branches.
+ *     time over a rolling register window (below);
+ *   - two-array wavefront kernel (n <= 64, nrhs <= 64, n + nrhs > 64): A's
+ *     and B's columns in separate register arrays of the same lanes.
  * The team kernel sums in row order as the reference does; the wavefront
  * kernel keeps four partial sums (rows i mod 4). Results agree with the
  * reference to rounding, which the tests bound.
@@ -246,6 +248,134 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
     }
 }
 
+/* ---- wavefront kernel, two arrays: n <= 64, nrhs <= 64, m <= 64 ------------
+ * For n + nrhs > 64 the columns of [A | B] outnumber the lanes: lane c holds
+ * A's column c (c < n) in f and B's column c (c < nrhs) in fb. Both take the
+ * same reflectors (one pair of v_readlane per row serves both), with the
+ * rolling windows of gels_wave_kernel. */
+template <class T>
+__global__ __launch_bounds__(256) void gels_wave2_kernel(BatchArgs<T> p)
+{
+    constexpr int MMAX = 64;
+    const int c = threadIdx.x & 63;
+    for (;;) {
+        unsigned kq = 0;
+        if (c == 0) kq = atomicAdd(p.counter, 1u);
+        kq = __builtin_amdgcn_readfirstlane(kq);
+        if (kq >= (unsigned)p.count) return;
+        const BatchProblem d = p.desc[kq];
+        const int m = d.m, n = d.n;
+        const bool isA = c < n, isB = c < d.nrhs;
+        T *acol = p.a + d.a_off + (long long)(isA ? c : 0) * d.lda;
+        T *bcol = p.b + d.b_off + (long long)(isB ? c : 0) * d.ldb;
+
+        T f[MMAX], fb[MMAX];
+        sfor<0, MMAX>([&](auto i) {
+            f[i] = T(0);
+            fb[i] = T(0);
+        });
+        if (isA) {
+            sfor<0, MMAX>([&](auto i) {
+                if (i < m) f[i] = acol[i];
+            });
+        }
+        if (isB) {
+            sfor<0, MMAX>([&](auto i) {
+                if (i < m) fb[i] = bcol[i];
+            });
+        }
+
+        T own = T(0);
+        for (int kk = 0; kk < n; ++kk) {
+            T tl = T(0);
+            sfor<1, MMAX>([&](auto i) { tl = fmaT(f[i], f[i], tl); });
+            const Larfg<T> h = larfg(rdlane(f[0], kk), rdlane(tl, kk));
+            if (c == kk) {
+                sfor<1, MMAX>([&](auto i) { f[i] = f[i] * h.inv; });
+                f[0] = h.rdiag;
+            }
+            T acc[4] = {f[0], T(0), T(0), T(0)};
+            T accb[4] = {fb[0], T(0), T(0), T(0)};
+            sfor<1, MMAX>([&](auto i) {
+                const T fi = (i > 4) ? after(f[i], acc[i % 4]) : f[i];
+                const T v = rdlane(fi, kk);
+                acc[i % 4] = fmaT(v, fi, acc[i % 4]);
+                accb[i % 4] = fmaT(v, fb[i], accb[i % 4]);
+            });
+            T wv = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            T wb = (accb[0] + accb[1]) + (accb[2] + accb[3]);
+            wv = (c > kk) ? wv * h.tau : T(0);
+            wb = isB ? wb * h.tau : T(0);
+            f[0] = f[0] - wv;
+            fb[0] = fb[0] - wb;
+            sfor<1, MMAX>([&](auto i) {
+                const T fi = after(f[i], (i > 4) ? f[i - 4] : wv);
+                const T v = rdlane(fi, kk);
+                f[i] = fmaT(-v, wv, fi);
+                fb[i] = fmaT(-v, wb, fb[i]);
+            });
+            own = (c == kk) ? f[0] : own;
+            if (isA) acol[kk] = f[0];
+            if (isB) bcol[kk] = fb[0];
+            sfor<0, MMAX - 1>([&](auto i) {
+                f[i] = f[i + 1];
+                fb[i] = fb[i + 1];
+            });
+            f[MMAX - 1] = T(0);
+            fb[MMAX - 1] = T(0);
+        }
+        sfor<0, MMAX>([&](auto i) {
+            if (i < m - n) {
+                if (isA) acol[n + i] = f[i];
+                if (isB) bcol[n + i] = fb[i];
+            }
+        });
+
+        int bad = 0;
+        for (int l = 0; l < n; ++l) {
+            if (rdlane(own, l) == T(0)) {
+                bad = l + 1;
+                break;
+            }
+        }
+        if (c == 0 && p.info) p.info[d.idx] = bad;
+        if (bad != 0) continue;
+
+        /* R X = (Q^T B)(0:n): g (R's columns) and gb (the right-hand sides),
+         * element j = row l - j at step l */
+        T g[MMAX], gb[MMAX];
+        sfor<0, MMAX>([&](auto j) {
+            g[j] = T(0);
+            gb[j] = T(0);
+        });
+        if (isA) {
+            sfor<0, MMAX>([&](auto j) {
+                if (j < n) g[j] = acol[n - 1 - j];
+            });
+        }
+        if (isB) {
+            sfor<0, MMAX>([&](auto j) {
+                if (j < n) gb[j] = bcol[n - 1 - j];
+            });
+        }
+        for (int l = n - 1; l >= 0; --l) {
+            const T dl = rdlane(own, l);
+            const T x = isB ? gb[0] / dl : T(0);
+            if (isB) bcol[l] = x;
+            sfor<1, MMAX>([&](auto j) {
+                const T gj = after(g[j], (j > 4) ? gb[j - 4] : x);
+                gb[j] = fmaT(-rdlane(gj, l), x, gb[j]);
+            });
+            sfor<0, MMAX - 1>([&](auto j) {
+                g[j] = g[j + 1];
+                gb[j] = gb[j + 1];
+            });
+            g[MMAX - 1] = T(0);
+            gb[MMAX - 1] = T(0);
+        }
+    }
+}
+
 template <class T> hipError_t launch_class(int cls, const BatchArgs<T> &args, unsigned blocks,
                                            hipStream_t s)
 {
@@ -256,6 +386,7 @@ template <class T> hipError_t launch_class(int cls, const BatchArgs<T> &args, un
     case 2: hipLaunchKernelGGL((gels_team_kernel<T, 64>), grid, block, 0, s, args); break;
     case 3: hipLaunchKernelGGL((gels_wave_kernel<T, 32>), grid, block, 0, s, args); break;
     case 4: hipLaunchKernelGGL((gels_wave_kernel<T, 64>), grid, block, 0, s, args); break;
+    case 5: hipLaunchKernelGGL((gels_wave2_kernel<T>), grid, block, 0, s, args); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -278,6 +409,7 @@ template <class T> hipError_t gels_batch_occupancy(int cls, int *blocks)
     case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_team_kernel<T, 64>, 256, 0);
     case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_kernel<T, 32>, 256, 0);
     case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_kernel<T, 64>, 256, 0);
+    case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave2_kernel<T>, 256, 0);
     default: *blocks = 0; return hipErrorInvalidValue;
     }
 }

@@ -70,7 +70,7 @@ int classify(const cbk_gels_problem &p, bool tran, const cbk_tuning &t)
     const int mb = p.m <= 16 ? 0 : p.m <= 32 ? 1 : 2;
     if (w <= 16 && w <= t.narrow_max_cols[mb]) return mb;
     if (w <= 64) return p.m <= 32 ? 3 : 4;
-    return kBatchGeneral;
+    return p.nrhs <= 64 ? 5 : kBatchGeneral;
 }
 
 /* Householder QR plus Q^T B plus the solve, in multiply-adds */
@@ -80,7 +80,7 @@ double work_of(const cbk_gels_problem &p)
     return m * n * n - n * n * n / 3.0 + 2.0 * m * n * r + n * n * r;
 }
 
-int problems_per_block(int cls) { return cls <= 2 ? 16 : cls <= 4 ? 4 : 256; }
+int problems_per_block(int cls) { return cls <= 2 ? 16 : cls <= 5 ? 4 : 256; }
 
 } /* namespace */
 

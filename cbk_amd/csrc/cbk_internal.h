@@ -113,9 +113,10 @@ template <class T> struct BatchArgs {
 };
 
 /* classes: 0-2 team kernel with 16/32/64 row registers, 3-4 wavefront kernel
- * with 32/64, 5 the general lane kernel */
-constexpr int kBatchClasses = 6;
-constexpr int kBatchGeneral = 5;
+ * with 32/64, 5 the two-array wavefront kernel (n + nrhs > 64), 6 the general
+ * lane kernel */
+constexpr int kBatchClasses = 7;
+constexpr int kBatchGeneral = 6;
 template <class T>
 hipError_t launch_gels_batch(int cls, const BatchArgs<T> &args, unsigned blocks, hipStream_t s);
 template <class T>

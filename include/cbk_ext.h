@@ -134,11 +134,13 @@ int cbk_gels_batch_plan(char prec, char trans, int nprob, const cbk_gels_problem
 size_t cbk_batch_plan_worksize(cbk_batch_plan_t plan);
 
 /* problems per kernel class (count[CBK_BATCH_NCLASS]); returns the number
- * of classes. Classes: 0-2 one problem per 16-lane team (rows <= 16, 32,
- * 64), 3-4 one problem per wavefront (rows <= 32, 64), 5 the general
- * one-lane-per-problem kernel (transposed or underdetermined problems, more
- * than 64 rows or n + nrhs > 64 columns, empty problems). */
-#define CBK_BATCH_NCLASS 6
+ * of classes. Classes: 0-2 one problem per 16-lane team (n + nrhs <= 16;
+ * rows <= 16, 32, 64), 3-4 one problem per wavefront (n + nrhs <= 64; rows
+ * <= 32, 64), 5 one problem per wavefront with A and B in separate registers
+ * (n + nrhs > 64, rows <= 64, nrhs <= 64), 6 the general one-lane-per-problem
+ * kernel (transposed or underdetermined problems, more than 64 rows, empty
+ * problems). */
+#define CBK_BATCH_NCLASS 7
 int cbk_batch_plan_classes(cbk_batch_plan_t plan, int *count);
 
 /* Codes: -1 plan NULL or of the other precision, -2 ap, -3 bp, -5 work NULL
