@@ -157,6 +157,21 @@ template <class T> __device__ __forceinline__ T after(T x, T dep)
     return x;
 }
 
+/* body(i) for the compile-time rows i of [B, E) whose 8-row chunk starts
+ * below `live` (wave-uniform): windows shrink as the reflectors advance, and
+ * the rows past the live ones are zeros that the skipped updates would leave
+ * zero */
+template <int B, int E, class F> __device__ __forceinline__ void live_rows(int live, F &&body)
+{
+    sfor<0, (E + 7) / 8>([&](auto ch) {
+        constexpr int lo = ch * 8 > B ? ch * 8 : B;
+        constexpr int hi = ch * 8 + 8 < E ? ch * 8 + 8 : E;
+        if constexpr (lo < hi) {
+            if (ch * 8 < live) sfor<lo, hi>(body);
+        }
+    });
+}
+
 template <class T, int MMAX>
 __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
 {
@@ -183,29 +198,30 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
 
         T own = T(0); /* R(c,c) of an A lane */
         for (int kk = 0; kk < n; ++kk) {
+            const int live = m - kk; /* window rows that can be nonzero */
             T tl = T(0);
-            sfor<1, MMAX>([&](auto i) { tl = fmaT(f[i], f[i], tl); });
+            live_rows<1, MMAX>(live, [&](auto i) { tl = fmaT(f[i], f[i], tl); });
             const Larfg<T> h = larfg(rdlane(f[0], kk), rdlane(tl, kk));
             if (c == kk) {
-                sfor<1, MMAX>([&](auto i) { f[i] = f[i] * h.inv; });
+                live_rows<1, MMAX>(live, [&](auto i) { f[i] = f[i] * h.inv; });
                 f[0] = h.rdiag;
             }
             /* w = f[0] + sum v_i f[i], four partial sums */
             T acc[4] = {f[0], T(0), T(0), T(0)};
-            sfor<1, MMAX>([&](auto i) {
+            live_rows<1, MMAX>(live, [&](auto i) {
                 const T fi = (i > 4) ? after(f[i], acc[i % 4]) : f[i];
                 acc[i % 4] = fmaT(rdlane(fi, kk), fi, acc[i % 4]);
             });
             T wv = (acc[0] + acc[1]) + (acc[2] + acc[3]);
             wv = (c > kk) ? wv * h.tau : T(0);
             f[0] = f[0] - wv;
-            sfor<1, MMAX>([&](auto i) {
+            live_rows<1, MMAX>(live, [&](auto i) {
                 const T fi = after(f[i], (i > 4) ? f[i - 4] : wv);
                 f[i] = fmaT(-rdlane(fi, kk), wv, fi);
             });
             own = (c == kk) ? f[0] : own;
             if (has) col[kk] = f[0];
-            sfor<0, MMAX - 1>([&](auto i) { f[i] = f[i + 1]; });
+            live_rows<0, MMAX - 1>(live, [&](auto i) { f[i] = f[i + 1]; });
             f[MMAX - 1] = T(0);
         }
         if (has) {
@@ -238,11 +254,11 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
             const T dl = rdlane(own, l);
             const T x = isB ? g[0] / dl : T(0);
             if (isB) col[l] = x;
-            sfor<1, MMAX>([&](auto j) {
+            live_rows<1, MMAX>(l + 1, [&](auto j) {
                 const T gj = after(g[j], (j > 4) ? g[j - 4] : x);
                 g[j] = fmaT(-rdlane(gj, l), x, gj);
             });
-            sfor<0, MMAX - 1>([&](auto j) { g[j] = g[j + 1]; });
+            live_rows<0, MMAX - 1>(l + 1, [&](auto j) { g[j] = g[j + 1]; });
             g[MMAX - 1] = T(0);
         }
     }
@@ -287,16 +303,17 @@ __global__ __launch_bounds__(256) void gels_wave2_kernel(BatchArgs<T> p)
 
         T own = T(0);
         for (int kk = 0; kk < n; ++kk) {
+            const int live = m - kk;
             T tl = T(0);
-            sfor<1, MMAX>([&](auto i) { tl = fmaT(f[i], f[i], tl); });
+            live_rows<1, MMAX>(live, [&](auto i) { tl = fmaT(f[i], f[i], tl); });
             const Larfg<T> h = larfg(rdlane(f[0], kk), rdlane(tl, kk));
             if (c == kk) {
-                sfor<1, MMAX>([&](auto i) { f[i] = f[i] * h.inv; });
+                live_rows<1, MMAX>(live, [&](auto i) { f[i] = f[i] * h.inv; });
                 f[0] = h.rdiag;
             }
             T acc[4] = {f[0], T(0), T(0), T(0)};
             T accb[4] = {fb[0], T(0), T(0), T(0)};
-            sfor<1, MMAX>([&](auto i) {
+            live_rows<1, MMAX>(live, [&](auto i) {
                 const T fi = (i > 4) ? after(f[i], acc[i % 4]) : f[i];
                 const T v = rdlane(fi, kk);
                 acc[i % 4] = fmaT(v, fi, acc[i % 4]);
@@ -308,7 +325,7 @@ __global__ __launch_bounds__(256) void gels_wave2_kernel(BatchArgs<T> p)
             wb = isB ? wb * h.tau : T(0);
             f[0] = f[0] - wv;
             fb[0] = fb[0] - wb;
-            sfor<1, MMAX>([&](auto i) {
+            live_rows<1, MMAX>(live, [&](auto i) {
                 const T fi = after(f[i], (i > 4) ? f[i - 4] : wv);
                 const T v = rdlane(fi, kk);
                 f[i] = fmaT(-v, wv, fi);
@@ -317,7 +334,7 @@ __global__ __launch_bounds__(256) void gels_wave2_kernel(BatchArgs<T> p)
             own = (c == kk) ? f[0] : own;
             if (isA) acol[kk] = f[0];
             if (isB) bcol[kk] = fb[0];
-            sfor<0, MMAX - 1>([&](auto i) {
+            live_rows<0, MMAX - 1>(live, [&](auto i) {
                 f[i] = f[i + 1];
                 fb[i] = fb[i + 1];
             });
@@ -362,11 +379,11 @@ __global__ __launch_bounds__(256) void gels_wave2_kernel(BatchArgs<T> p)
             const T dl = rdlane(own, l);
             const T x = isB ? gb[0] / dl : T(0);
             if (isB) bcol[l] = x;
-            sfor<1, MMAX>([&](auto j) {
+            live_rows<1, MMAX>(l + 1, [&](auto j) {
                 const T gj = after(g[j], (j > 4) ? gb[j - 4] : x);
                 gb[j] = fmaT(-rdlane(gj, l), x, gb[j]);
             });
-            sfor<0, MMAX - 1>([&](auto j) {
+            live_rows<0, MMAX - 1>(l + 1, [&](auto j) {
                 g[j] = g[j + 1];
                 gb[j] = gb[j + 1];
             });
