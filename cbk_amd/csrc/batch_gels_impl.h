@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void gels_team_kernel(BatchArgs<T> p)
             if (l < n && bad == 0 && dl == T(0)) bad = l + 1;
         });
         if (live && c == 0 && p.info) p.info[d.idx] = bad;
-        if (bad == 0) {
+        if (bad == 0 || !p.info) {
             sfor_down<0, TS>([&](auto lc) {
                 constexpr int l = lc;
                 if (l < n) {
@@ -157,6 +157,16 @@ template <class T> __device__ __forceinline__ T after(T x, T dep)
     return x;
 }
 
+/* the problem the wavefront took: from the table, or computed in compact mode */
+template <class T>
+__device__ __forceinline__ BatchProblem problem_at(const BatchArgs<T> &p, unsigned k)
+{
+    if (p.desc) return p.desc[k];
+    const long long g = k / p.cV, v = k % p.cV;
+    return BatchProblem{p.cm, p.cn, p.cnrhs, p.a_cs, p.b_cs, (int)k,
+                        g * p.a_gs + v, g * p.b_gs + v, 0};
+}
+
 /* body(i) for the compile-time rows i of [B, E) whose 8-row chunk starts
  * below `live` (wave-uniform): windows shrink as the reflectors advance, and
  * the rows past the live ones are zeros that the skipped updates would leave
@@ -176,23 +186,32 @@ template <class T, int MMAX>
 __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
 {
     const int c = threadIdx.x & 63;
-    for (;;) {
+    for (int round = 0;; ++round) {
         unsigned kq = 0;
-        if (c == 0) kq = atomicAdd(p.counter, 1u);
-        kq = __builtin_amdgcn_readfirstlane(kq);
+        if (p.counter) {
+            if (c == 0) kq = atomicAdd(p.counter, 1u);
+            kq = __builtin_amdgcn_readfirstlane(kq);
+        }
+        else {
+            /* compact mode: one problem per wavefront */
+            if (round > 0) return;
+            kq = blockIdx.x * 4u + (unsigned)wave_in_block();
+        }
         if (kq >= (unsigned)p.count) return;
-        const BatchProblem d = p.desc[kq];
+        const BatchProblem d = problem_at(p, kq);
         const int m = d.m, n = d.n, w = d.n + d.nrhs;
         const bool isA = c < n, isB = c >= n && c < w;
         const bool has = isA || isB;
+        const long long rs = isA ? p.a_rs : p.b_rs;
         T *col = isA ? p.a + d.a_off + (long long)c * d.lda
                      : p.b + d.b_off + (long long)(isB ? c - n : 0) * d.ldb;
+        T *tau = p.ctau ? p.ctau + (long long)(kq / p.cV) * p.cn * p.cV + kq % p.cV : nullptr;
 
         T f[MMAX];
         sfor<0, MMAX>([&](auto i) { f[i] = T(0); });
         if (has) {
             sfor<0, MMAX>([&](auto i) {
-                if (i < m) f[i] = col[i];
+                if (i < m) f[i] = col[i * rs];
             });
         }
 
@@ -202,6 +221,7 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
             T tl = T(0);
             live_rows<1, MMAX>(live, [&](auto i) { tl = fmaT(f[i], f[i], tl); });
             const Larfg<T> h = larfg(rdlane(f[0], kk), rdlane(tl, kk));
+            if (tau && c == 0) tau[(long long)kk * p.cV] = h.tau;
             if (c == kk) {
                 live_rows<1, MMAX>(live, [&](auto i) { f[i] = f[i] * h.inv; });
                 f[0] = h.rdiag;
@@ -220,14 +240,14 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
                 f[i] = fmaT(-rdlane(fi, kk), wv, fi);
             });
             own = (c == kk) ? f[0] : own;
-            if (has) col[kk] = f[0];
+            if (has) col[kk * rs] = f[0];
             live_rows<0, MMAX - 1>(live, [&](auto i) { f[i] = f[i + 1]; });
             f[MMAX - 1] = T(0);
         }
         if (has) {
             /* rows n..m-1: Q^T B below the solution, reflector bodies of A */
             sfor<0, MMAX>([&](auto i) {
-                if (i < m - n) col[n + i] = f[i];
+                if (i < m - n) col[(n + i) * rs] = f[i];
             });
         }
 
@@ -239,21 +259,21 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
                 break;
             }
         }
-        if (c == 0 && p.info) p.info[d.idx] = bad;
-        if (bad != 0) continue;
+        if (c == 0 && p.info && d.idx < p.ninfo) p.info[d.idx] = bad;
+        if (bad != 0 && p.info) continue; /* LAPACK dgels stops; without info, solve as gels_compact */
 
         /* R X = (Q^T B)(0:n): g[j] = row l - j of the lane's column at step l */
         T g[MMAX];
         sfor<0, MMAX>([&](auto j) { g[j] = T(0); });
         if (has) {
             sfor<0, MMAX>([&](auto j) {
-                if (j < n) g[j] = col[n - 1 - j];
+                if (j < n) g[j] = col[(n - 1 - j) * rs];
             });
         }
         for (int l = n - 1; l >= 0; --l) {
             const T dl = rdlane(own, l);
             const T x = isB ? g[0] / dl : T(0);
-            if (isB) col[l] = x;
+            if (isB) col[l * rs] = x;
             live_rows<1, MMAX>(l + 1, [&](auto j) {
                 const T gj = after(g[j], (j > 4) ? g[j - 4] : x);
                 g[j] = fmaT(-rdlane(gj, l), x, gj);
@@ -356,7 +376,7 @@ __global__ __launch_bounds__(256) void gels_wave2_kernel(BatchArgs<T> p)
             }
         }
         if (c == 0 && p.info) p.info[d.idx] = bad;
-        if (bad != 0) continue;
+        if (bad != 0 && p.info) continue; /* LAPACK dgels stops; without info, solve as gels_compact */
 
         /* R X = (Q^T B)(0:n): g (R's columns) and gb (the right-hand sides),
          * element j = row l - j at step l */
@@ -415,6 +435,47 @@ template <class T>
 hipError_t launch_gels_batch(int cls, const BatchArgs<T> &args, unsigned blocks, hipStream_t s)
 {
     return batch::launch_class<T>(cls, args, blocks, s);
+}
+
+template <class T>
+hipError_t launch_qr_wave(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
+                          Operand<T> B, int V, int nm, int *info)
+{
+    BatchArgs<T> args;
+    int si = A.row ? A.ld : 1, sj = A.row ? 1 : A.ld;
+    if (transposed) {
+        const int tmp = si;
+        si = sj;
+        sj = tmp;
+    }
+    args.a = A.p;
+    args.b = B.p;
+    args.info = info;
+    args.desc = nullptr;
+    args.counter = nullptr;
+    args.count = ((nm + V - 1) / V) * V; /* padded slots too, as the reference */
+    args.tau = nullptr;
+    args.tran = false;
+    args.a_rs = si * V;
+    args.a_cs = sj * V;
+    args.b_rs = (B.row ? B.ld : 1) * V;
+    args.b_cs = (B.row ? 1 : B.ld) * V;
+    args.ninfo = nm;
+    args.cm = p;
+    args.cn = q;
+    args.cnrhs = nrhs;
+    args.cV = V;
+    args.a_gs = (long long)A.gstride;
+    args.b_gs = (long long)B.gstride;
+    args.ctau = tau;
+    const unsigned blocks = (unsigned)((args.count + 3) / 4);
+    if (p <= 32)
+        hipLaunchKernelGGL((batch::gels_wave_kernel<T, 32>), dim3(blocks), dim3(256), 0,
+                           current_stream(), args);
+    else
+        hipLaunchKernelGGL((batch::gels_wave_kernel<T, 64>), dim3(blocks), dim3(256), 0,
+                           current_stream(), args);
+    return hipGetLastError();
 }
 
 template <class T> hipError_t gels_batch_occupancy(int cls, int *blocks)

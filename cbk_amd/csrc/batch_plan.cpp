@@ -141,6 +141,11 @@ BatchArgs<T> class_args(const cbk_batch_plan *pl, int c, T *ap, T *bp, int *info
     a.count = pl->begin[c + 1] - pl->begin[c];
     a.tau = reinterpret_cast<T *>(work + pl->tau_off);
     a.tran = pl->tran;
+    a.a_rs = a.b_rs = 1;
+    a.ninfo = 0x7fffffff;
+    a.cm = a.cn = a.cnrhs = a.cV = a.a_cs = a.b_cs = 0;
+    a.a_gs = a.b_gs = 0;
+    a.ctau = nullptr;
     return a;
 }
 

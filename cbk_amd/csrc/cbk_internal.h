@@ -110,6 +110,16 @@ template <class T> struct BatchArgs {
     int count;
     T *tau;   /* general kernel only */
     bool tran;
+    /* row strides in elements (1 for the column-major problems of a batch) */
+    int a_rs, b_rs;
+    int ninfo; /* info[idx] is written for idx < ninfo */
+    /* compact mode (desc == null, counter == null, wavefront kernel): problem k
+     * is cm x cn with cnrhs right-hand sides, element (i, c) of its A at
+     * a + (k / cV) * a_gs + k % cV + i * a_rs + c * a_cs (B likewise); the
+     * reflector scalars go to ctau[(k / cV) * cn * cV + kk * cV + k % cV] */
+    int cm, cn, cnrhs, cV, a_cs, b_cs;
+    long long a_gs, b_gs;
+    T *ctau;
 };
 
 /* classes: 0-2 team kernel with 16/32/64 row registers, 3-4 wavefront kernel
@@ -121,6 +131,12 @@ template <class T>
 hipError_t launch_gels_batch(int cls, const BatchArgs<T> &args, unsigned blocks, hipStream_t s);
 template <class T>
 hipError_t launch_gels_batch_lane(const BatchArgs<T> &args, hipStream_t s);
+/* gels on compact operands through the wavefront kernel: the tall view p x q
+ * (transposed: A^T), p <= 64, q + nrhs <= 64, nrhs >= 1, any V */
+bool qr_wave_supports(int p, int q, int nrhs);
+template <class T>
+hipError_t launch_qr_wave(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
+                          Operand<T> B, int V, int nm, int *info);
 /* resident 256-thread blocks per compute unit of class cls's kernel */
 template <class T> hipError_t gels_batch_occupancy(int cls, int *blocks);
 hipError_t launch_clear_u32(unsigned *p, int n, hipStream_t s);

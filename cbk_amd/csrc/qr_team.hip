@@ -16,6 +16,8 @@ CBK_QR_TEAM_EXTERN(float)
 extern template hipError_t launch_pq<float, 64, 32, 2>(const QrArgs<float> &);
 } /* namespace qrteam */
 
+bool qr_wave_supports(int p, int q, int nrhs) { return p <= 64 && nrhs >= 1 && q + nrhs <= 64; }
+
 bool qr_team_supports(int p, int q, int nrhs, bool fp64)
 {
     return nrhs <= 2 && ((p == 16 && q == 8) || (p == 32 && q == 16) ||
