@@ -26,6 +26,7 @@ ROUTINES = {
 # entry points of include/cbk_ext.h beyond the _info variants ('z' = size_t)
 EXT = {
     "dsposv_compact": "cciipipipipzppii",
+    "dsgels_compact": "cciiipipipippii",
 }
 
 # per-matrix info variants: the plain argument list plus a trailing device int[nm]

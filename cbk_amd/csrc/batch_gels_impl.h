@@ -198,6 +198,7 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
             kq = blockIdx.x * 4u + (unsigned)wave_in_block();
         }
         if (kq >= (unsigned)p.count) return;
+        if (p.mask && (kq >= (unsigned)p.ninfo || p.mask[kq] >= 0)) continue;
         const BatchProblem d = problem_at(p, kq);
         const int m = d.m, n = d.n, w = d.n + d.nrhs;
         const bool isA = c < n, isB = c >= n && c < w;
@@ -206,6 +207,11 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
         T *col = isA ? p.a + d.a_off + (long long)c * d.lda
                      : p.b + d.b_off + (long long)(isB ? c - n : 0) * d.ldb;
         T *tau = p.ctau ? p.ctau + (long long)(kq / p.cV) * p.cn * p.cV + kq % p.cV : nullptr;
+        /* where the B lanes' results go: B itself, or X (rows < n) */
+        const bool xout = isB && p.x;
+        T *ocol = xout ? p.x + (long long)(kq / p.cV) * p.x_gs + kq % p.cV + (long long)(c - n) * p.x_cs
+                       : col;
+        const long long ors = xout ? p.x_rs : rs;
 
         T f[MMAX];
         sfor<0, MMAX>([&](auto i) { f[i] = T(0); });
@@ -240,11 +246,11 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
                 f[i] = fmaT(-rdlane(fi, kk), wv, fi);
             });
             own = (c == kk) ? f[0] : own;
-            if (has) col[kk * rs] = f[0];
+            if (has) ocol[kk * ors] = f[0];
             live_rows<0, MMAX - 1>(live, [&](auto i) { f[i] = f[i + 1]; });
             f[MMAX - 1] = T(0);
         }
-        if (has) {
+        if (has && !xout) {
             /* rows n..m-1: Q^T B below the solution, reflector bodies of A */
             sfor<0, MMAX>([&](auto i) {
                 if (i < m - n) col[(n + i) * rs] = f[i];
@@ -267,13 +273,13 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
         sfor<0, MMAX>([&](auto j) { g[j] = T(0); });
         if (has) {
             sfor<0, MMAX>([&](auto j) {
-                if (j < n) g[j] = col[(n - 1 - j) * rs];
+                if (j < n) g[j] = ocol[(n - 1 - j) * ors];
             });
         }
         for (int l = n - 1; l >= 0; --l) {
             const T dl = rdlane(own, l);
             const T x = isB ? g[0] / dl : T(0);
-            if (isB) col[l * rs] = x;
+            if (isB) ocol[l * ors] = x;
             live_rows<1, MMAX>(l + 1, [&](auto j) {
                 const T gj = after(g[j], (j > 4) ? g[j - 4] : x);
                 g[j] = fmaT(-rdlane(gj, l), x, gj);
@@ -468,6 +474,10 @@ hipError_t launch_qr_wave(int p, int q, int nrhs, bool transposed, Operand<T> A,
     args.a_gs = (long long)A.gstride;
     args.b_gs = (long long)B.gstride;
     args.ctau = tau;
+    args.mask = nullptr;
+    args.x = nullptr;
+    args.x_rs = args.x_cs = 0;
+    args.x_gs = 0;
     const unsigned blocks = (unsigned)((args.count + 3) / 4);
     if (p <= 32)
         hipLaunchKernelGGL((batch::gels_wave_kernel<T, 32>), dim3(blocks), dim3(256), 0,

@@ -146,6 +146,10 @@ BatchArgs<T> class_args(const cbk_batch_plan *pl, int c, T *ap, T *bp, int *info
     a.cm = a.cn = a.cnrhs = a.cV = a.a_cs = a.b_cs = 0;
     a.a_gs = a.b_gs = 0;
     a.ctau = nullptr;
+    a.mask = nullptr;
+    a.x = nullptr;
+    a.x_rs = a.x_cs = 0;
+    a.x_gs = 0;
     return a;
 }
 

@@ -294,6 +294,41 @@ int dsposv(char layout, char uplo, int n, int nrhs, double *ap, int ldap, const 
                                          nm, kItermax, iter, info));
 }
 
+/* dsgels_compact (include/cbk_ext.h): the FP32-preconditioned refinement of
+ * mixed_gels_impl.h where a team kernel serves the shape, then the FP64
+ * wavefront gels on the matrices it marked (iter < 0), B read only */
+int dsgels(char layout, char trans, int m, int n, int nrhs, double *ap, int ldap,
+           const double *bp, int ldbp, double *xp, int ldxp, int *iter, int *info, int V, int nm)
+{
+    const bool col = opt(layout, 'C'), row = opt(layout, 'R');
+    if (!col && !row) return -1;
+    if (!opt(trans, 'N')) return -2;
+    if (m < 0 || m > 64) return -3;
+    if (n < 0 || n > m) return -4;
+    if (nrhs < 0 || n + nrhs > 64) return -5;
+    if (ldap < max1(row ? n : m)) return -7;
+    if (ldbp < max1(row ? nrhs : m)) return -9;
+    if (ldxp < max1(row ? nrhs : n)) return -11;
+    if (!vlen_ok(V)) return -14;
+    if (nm < 0) return -15;
+    hipError_t e = launch_clear_info(info, nm);
+    if (n == 0 || nrhs == 0 || nm == 0) {
+        if (e == hipSuccess) e = launch_clear_info(iter, nm);
+        return status(e);
+    }
+    const Operand<double> A = operand(ap, ldap, row, m, n, V);
+    const Operand<double> B = operand(const_cast<double *>(bp), ldbp, row, m, nrhs, V);
+    const Operand<double> X = operand(xp, ldxp, row, n, nrhs, V);
+    if (e == hipSuccess) {
+        if (current_policy() != CBK_POLICY_LANE && V >= 4 && mixed_gels_supports(m, n, nrhs))
+            e = launch_mixed_gels_team(m, n, nrhs, A, B, X, V, nm, kItermax, iter);
+        else
+            e = launch_fill_int(iter, nm, -1);
+    }
+    if (e == hipSuccess) e = launch_qr_wave_masked(m, n, nrhs, A, B, X, V, nm, iter, info);
+    return status(e);
+}
+
 } /* namespace */
 } /* namespace cbk */
 
@@ -417,6 +452,13 @@ int dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap
 {
     return dsposv(layout, uplo, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, work, lwork, iter,
                   info, V, nm);
+}
+
+int dsgels_compact(char layout, char trans, int m, int n, int nrhs, double *ap, int ldap,
+                   const double *bp, int ldbp, double *xp, int ldxp, int *iter, int *info,
+                   int V, int nm)
+{
+    return dsgels(layout, trans, m, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, iter, info, V, nm);
 }
 
 size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int V, int nm)

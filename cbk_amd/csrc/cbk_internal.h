@@ -120,6 +120,14 @@ template <class T> struct BatchArgs {
     int cm, cn, cnrhs, cV, a_cs, b_cs;
     long long a_gs, b_gs;
     T *ctau;
+    /* compact mode, optional: only problems k < ninfo with mask[k] < 0 are
+     * solved; with x set, B is read only and the first cn rows of the
+     * solution go to x (element (i, j) at x + (k / cV) * x_gs + k % cV +
+     * i * x_rs + j * x_cs) */
+    const int *mask;
+    T *x;
+    int x_rs, x_cs;
+    long long x_gs;
 };
 
 /* classes: 0-2 team kernel with 16/32/64 row registers, 3-4 wavefront kernel
@@ -137,6 +145,17 @@ bool qr_wave_supports(int p, int q, int nrhs);
 template <class T>
 hipError_t launch_qr_wave(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
                           Operand<T> B, int V, int nm, int *info);
+/* the same kernel as the FP64 path of dsgels_compact: matrices k with
+ * mask[k] < 0 only, B read only, X (q x nrhs) written; A receives the QR */
+hipError_t launch_qr_wave_masked(int p, int q, int nrhs, Operand<double> A, Operand<double> B,
+                                 Operand<double> X, int V, int nm, const int *mask, int *info);
+
+/* ---- mixed-precision least squares (mixed_gels_impl.h) ---- */
+bool mixed_gels_supports(int p, int q, int nrhs);
+hipError_t launch_mixed_gels_team(int p, int q, int nrhs, Operand<double> A, Operand<double> B,
+                                  Operand<double> X, int V, int nm, int itermax, int *iter);
+/* p[0:n] := value */
+hipError_t launch_fill_int(int *p, int n, int value);
 /* resident 256-thread blocks per compute unit of class cls's kernel */
 template <class T> hipError_t gels_batch_occupancy(int cls, int *blocks);
 hipError_t launch_clear_u32(unsigned *p, int n, hipStream_t s);

@@ -622,7 +622,7 @@ template <class T> hipError_t launch_zero(int rows, int cols, Operand<T> B, int 
 
 hipError_t launch_clear_info(int *info, int nm)
 {
-    if (nm <= 0) return hipSuccess;
+    if (nm <= 0 || !info) return hipSuccess;
     hipLaunchKernelGGL(clear_info_kernel, dim3((nm + kBlock - 1) / kBlock), dim3(kBlock), 0,
                        current_stream(), info, nm);
     return hipGetLastError();
@@ -668,6 +668,20 @@ __global__ void clear_u32_kernel(unsigned *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+__global__ void fill_int_kernel(int *p, int n, int value)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+hipError_t launch_fill_int(int *p, int n, int value)
+{
+    if (n <= 0 || !p) return hipSuccess;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       current_stream(), p, n, value);
+    return hipGetLastError();
 }
 
 hipError_t launch_clear_u32(unsigned *p, int n, hipStream_t s)

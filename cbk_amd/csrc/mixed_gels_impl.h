@@ -1,0 +1,227 @@
+/* mixed_gels_impl.h -- mixed-precision least squares (dsgels_compact), one
+ * P x Q matrix per 16-lane team.
+ *
+ * The FP32 Householder QR of A (the team QR of qr_team_impl.h, on the FP32
+ * rounding of A) gives R with R^T R = A^T A + E, ||E|| ~ eps32 ||A||^2. R then
+ * preconditions iterative refinement on the normal equations, whose residuals
+ * are computed in FP64 from the FP64 A and B read from HBM:
+ *     x_0 = 0;  r = b - A x,  g = A^T r      (FP64, the only FP64 arithmetic)
+ *               R^T R d = g                  (FP32: two triangular sweeps)
+ *               x += d                       (FP64)
+ * The fixed point is A^T (b - A x) = 0 in FP64, the least-squares solution, so
+ * X reaches FP64 accuracy while the factorization and the sweeps run in FP32;
+ * each step contracts the error by about kappa(A)^2 eps32. Per column the
+ * loop stops when ||d||_inf <= ||x||_inf * 2^-53 * sqrt(Q) (the dsposv test on
+ * the correction), or when the correction no longer halves and is already
+ * below ||x||_inf * 2^-40 (the rounding floor of A^T r, which kappa^2 lifts
+ * above 2^-53 for inconsistent systems); a matrix whose FP32 R has a zero or
+ * non-finite diagonal, whose correction grows or stalls above that floor, or
+ * that runs ITERMAX steps is marked for the FP64 path (iter < 0), which a
+ * second launch (the wavefront gels kernel on those matrices only) runs.
+ *
+ * Layout in the team: lane t owns rows t + 16 r of A (as the QR team kernel);
+ * after the QR only R's rows (r < Q/16) are kept, FP32. The sweeps work on a
+ * right-hand side replicated in every lane of the team (g, then h, then d):
+ *   R^T h = g : h_k = g_k / R(k,k); g_i -= R(k,i) h_k, R(k,i) broadcast from
+ *               row k's lane (fused DPP fma)
+ *   R d = h   : d_i = (h_i - sum_{k>i} R(i,k) d_k) / R(i,i) in row i's lane,
+ *               then broadcast
+ * and x stays distributed like the rows (lane t: x_t, x_{t+16}).
+ */
+#ifndef CBK_MIXED_GELS_IMPL_H
+#define CBK_MIXED_GELS_IMPL_H
+
+#include "cbk_internal.h"
+#include "team_common.h"
+#include "team_chol.h"
+
+namespace cbk {
+namespace mixedgels {
+
+using namespace team;
+
+struct MixedGelsArgs {
+    const double *a; /* p x q, element (i, c) at (i*a_si + c*a_sj)*V in the group */
+    int a_si, a_sj;
+    size_t a_gs;
+    const double *b; /* p x nrhs */
+    int b_si, b_sj;
+    size_t b_gs;
+    double *x; /* q x nrhs */
+    int x_si, x_sj;
+    size_t x_gs;
+    int V;
+    long nmat;
+    int nm;
+    int nrhs;
+    int itermax;
+    int *iter;
+};
+
+constexpr int kTeamsPerBlock = 4;
+
+template <class T> __device__ __forceinline__ T team_sum(T x)
+{
+    x = x + dpp<0xB1>(x);
+    x = x + dpp<0x4E>(x);
+    x = x + dpp<0x141>(x);
+    x = x + dpp<0x140>(x);
+    return x;
+}
+
+template <int P, int Q>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
+mixed_gels_kernel(MixedGelsArgs p)
+{
+    constexpr int R = P / TS;  /* row blocks of A */
+    constexpr int RQ = (Q + TS - 1) / TS; /* row blocks of R */
+    constexpr unsigned SZ = 8;
+    const int t = threadIdx.x & (TS - 1);
+    const int team = threadIdx.x / TS;
+    const long mat0 = (long)xcd_block(blockIdx.x) * kTeamsPerBlock + 4 * wave_in_block();
+    if (mat0 >= p.nmat) return;
+    const long mat = mat0 + (team & 3);
+    const long g = mat0 / p.V;
+    const long groups_left = p.nmat / p.V - g;
+    const unsigned v = (unsigned)(mat0 % p.V) + (team & 3);
+    const unsigned V = (unsigned)p.V;
+    const __amdgpu_buffer_rsrc_t ra = group_rsrc(p.a + g * p.a_gs, group_range(groups_left, p.a_gs, SZ));
+    const __amdgpu_buffer_rsrc_t rb = group_rsrc(p.b + g * p.b_gs, group_range(groups_left, p.b_gs, SZ));
+    const __amdgpu_buffer_rsrc_t rx = group_rsrc(p.x + g * p.x_gs, group_range(groups_left, p.x_gs, SZ));
+    const unsigned a_col = (unsigned)p.a_sj * V * SZ, b_col = (unsigned)p.b_sj * V * SZ,
+                   x_col = (unsigned)p.x_sj * V * SZ;
+    unsigned a_row[R], b_row[R], x_row[RQ];
+    sfor<0, R>([&](auto r) {
+        const int i = t + TS * r;
+        a_row[r] = ((unsigned)(i * p.a_si) * V + v) * SZ;
+        b_row[r] = ((unsigned)(i * p.b_si) * V + v) * SZ;
+    });
+    sfor<0, RQ>([&](auto r) {
+        const int i = t + TS * r;
+        x_row[r] = (i < Q) ? ((unsigned)(i * p.x_si) * V + v) * SZ : kOOB;
+    });
+
+    /* ---- FP32 QR of A (geqr2, qr_team_impl.h), R only ---- */
+    float rr[RQ][Q];
+    bool ok = true;
+    {
+        float f[R][Q];
+        sfor<0, R>([&](auto r) {
+            sfor<0, Q>([&](auto c) { f[r][c] = (float)bload(0.0, ra, a_row[r], c * a_col); });
+        });
+        sfor<0, Q>([&](auto kc) {
+            constexpr int kk = kc;
+            constexpr int rk = kk / TS, tk = kk % TS;
+            float part = 0.f;
+            sfor<rk, R>([&](auto r) {
+                const int i = t + TS * r;
+                const float xv = (i > kk) ? f[r][kk] : 0.f;
+                part = __builtin_fmaf(xv, xv, part);
+            });
+            const float tail = team_sum(part);
+            const float x0 = bcast<tk>(f[rk][kk]);
+            const float norm = __builtin_sqrtf(__builtin_fmaf(x0, x0, tail));
+            const float beta = (x0 >= 0.f) ? -norm : norm;
+            const bool has = tail > 0.f;
+            const float tau = has ? (beta - x0) / beta : 0.f;
+            const float inv = has ? 1.f / (x0 - beta) : 0.f;
+            const float rdiag = has ? beta : x0;
+            float vi[R];
+            sfor<rk, R>([&](auto r) {
+                const int i = t + TS * r;
+                f[r][kk] = (i > kk) ? f[r][kk] * inv : (i == kk ? rdiag : f[r][kk]);
+                vi[r] = (i > kk) ? f[r][kk] : (i == kk ? 1.f : 0.f);
+            });
+            sfor<kk + 1, Q>([&](auto c) {
+                float pw = 0.f;
+                sfor<rk, R>([&](auto r) { pw = __builtin_fmaf(vi[r], f[r][c], pw); });
+                const float w = team_sum(pw) * tau;
+                sfor<rk, R>([&](auto r) { f[r][c] = __builtin_fmaf(-vi[r], w, f[r][c]); });
+            });
+            __builtin_amdgcn_sched_barrier(0);
+        });
+        /* keep R (rows < Q, upper triangle); a zero or non-finite diagonal
+         * sends the matrix to the FP64 path */
+        sfor<0, RQ>([&](auto r) {
+            const int i = t + TS * r;
+            sfor<0, Q>([&](auto c) { rr[r][c] = (c >= i) ? f[r][c] : 0.f; });
+        });
+        sfor<0, Q>([&](auto kc) {
+            const float d = bcast<kc % TS>(rr[kc / TS][kc]);
+            ok = ok && (d != 0.f) && __builtin_isfinite(d);
+        });
+    }
+
+    int steps = 0;
+    const double cte = 0x1p-53 * __builtin_sqrt((double)Q), floor_ = 0x1p-40;
+    for (int q = 0; q < p.nrhs && ok; ++q) {
+        double x[RQ];
+        sfor<0, RQ>([&](auto r) { x[r] = 0.0; });
+        double prev = __builtin_inf();
+        bool done = false;
+        for (int it = 0; it <= p.itermax; ++it) {
+            /* g = A^T (b - A x), FP64, one row block of A at a time */
+            double gp[Q];
+            sfor<0, Q>([&](auto c) { gp[c] = 0.0; });
+            sfor<0, R>([&](auto r) {
+                double arow[Q];
+                sfor<0, Q>([&](auto c) { arow[c] = bload(0.0, ra, a_row[r], c * a_col); });
+                double res = bload(0.0, rb, b_row[r], q * b_col);
+                sfor<0, Q>([&](auto c) { fnma_bc<c % TS>(res, x[c / TS], arow[c]); });
+                sfor<0, Q>([&](auto c) { gp[c] = __builtin_fma(arow[c], res, gp[c]); });
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            float h[Q];
+            sfor<0, Q>([&](auto c) { h[c] = (float)team_sum(gp[c]); });
+            /* R^T h = g */
+            sfor<0, Q>([&](auto kc) {
+                constexpr int k = kc;
+                const float dk = bcast<k % TS>(rr[k / TS][k]);
+                h[k] = h[k] / dk;
+                sfor<k + 1, Q>([&](auto i) { fnma_bc<k % TS>(h[i], rr[k / TS][i], h[k]); });
+            });
+            /* R d = h, d overwriting h from the bottom */
+            sfor_down<0, Q>([&](auto ic) {
+                constexpr int i = ic;
+                float s = h[i];
+                sfor<i + 1, Q>([&](auto k) { s = __builtin_fmaf(-rr[i / TS][k], h[k], s); });
+                s = s / rr[i / TS][i];
+                h[i] = bcast<i % TS>(s);
+            });
+            double dn = 0.0;
+            sfor<0, Q>([&](auto i) {
+                dn = fmax_nan(dn, (double)__builtin_fabsf(h[i]));
+                x[i / TS] = x[i / TS] + ((t == i % TS) ? (double)h[i] : 0.0);
+            });
+            double xn = 0.0;
+            sfor<0, RQ>([&](auto r) { xn = fmax_nan(xn, __builtin_fabs(x[r])); });
+            xn = team_max(xn);
+            steps = it > steps ? it : steps;
+            if (dn <= xn * cte || (it > 0 && dn > 0.5 * prev && dn <= xn * floor_)) {
+                done = true;
+                break;
+            }
+            if (!(dn < prev) && it > 0) break; /* not contracting: FP64 path */
+            prev = dn;
+        }
+        if (!done) {
+            ok = false;
+            break;
+        }
+        sfor<0, RQ>([&](auto r) { bstore(x[r], rx, x_row[r], q * x_col); });
+    }
+    if (mat < p.nm && t == 0) p.iter[mat] = ok ? steps : -p.itermax - 1;
+}
+
+template <int P, int Q> hipError_t launch_pq(const MixedGelsArgs &args)
+{
+    const unsigned blocks = (unsigned)((args.nmat + kTeamsPerBlock - 1) / kTeamsPerBlock);
+    hipLaunchKernelGGL((mixed_gels_kernel<P, Q>), dim3(xcd_grid(blocks)), dim3(64), 0,
+                       current_stream(), args);
+    return hipGetLastError();
+}
+
+} /* namespace mixedgels */
+} /* namespace cbk */
+
+#endif /* CBK_MIXED_GELS_IMPL_H */

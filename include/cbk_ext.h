@@ -95,6 +95,31 @@ int dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap
 /* bytes of workspace dsposv_compact needs for this call shape (0 if none) */
 size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int V, int nm);
 
+/* dsgels_compact: mixed-precision least squares min ||B - A X|| for every
+ * matrix (trans 'N', n <= m <= 64, n + nrhs <= 64, full column rank). No LAPACK
+ * routine has this contract; it follows dsposv's. The FP32 Householder QR of
+ * A preconditions iterative refinement on the normal equations, whose
+ * residuals r = B - A X and A^T r are computed in FP64 (X_0 = 0, then X += d
+ * with R^T R d = A^T r solved in FP32), per column until
+ *     ||d||_inf <= ||x||_inf * eps * sqrt(n)      (eps = 2^-53)
+ * or until the correction stops halving once below ||x||_inf * 2^-40 (the
+ * rounding floor of A^T r for an inconsistent system). Per matrix:
+ *     iter  >= 0 refinement steps (the most any column took; A untouched);
+ *           -1 the shape has no FP32 kernel (m x n not 16 x 8, 32 x 16 or
+ *           64 x 32, nrhs > 8, or V < 4) and -31 refinement failed (a zero or
+ *           non-finite diagonal of the FP32 R, a correction that does not
+ *           contract, or 30 steps): the FP64 gels ran and A holds its QR
+ *           (reflectors below the diagonal, as gels_compact);
+ *     info  0, or (FP64 path) the 1-based index of the first exact zero of
+ *           R's diagonal, X then holding (Q^T B)(0:n).
+ * B (m x nrhs, ldbp) is read only; X (n x nrhs, ldxp, same layout) receives
+ * the solution. iter is required (device int[nm]); info may be NULL.
+ * Codes: -1 layout -2 trans -3 m -4 n -5 nrhs -7 ldap -9 ldbp -11 ldxp -14 V
+ *        -15 nm */
+int dsgels_compact(char layout, char trans, int m, int n, int nrhs, double *ap, int ldap,
+                   const double *bp, int ldbp, double *xp, int ldxp, int *iter, int *info,
+                   int V, int nm);
+
 /* ---- uneven batches -------------------------------------------------------
  * No reference counterpart. A batch of independent least-squares problems
  * min || B_k - op(A_k) X_k || of different sizes, each stored column-major on

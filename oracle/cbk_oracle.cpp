@@ -881,6 +881,142 @@ int gels_batch(char trans, int nprob, const oracle_gels_problem *desc, T *ap, T 
 
 extern "C" {
 
+/* Mixed-precision least squares (no reference counterpart): the single
+ * precision Householder QR (this file's geqrf_lane, the reference's geqr2 in
+ * float) preconditions refinement on the normal equations with double
+ * residuals:  x = 0;  r = b - A x, g = A^T r (double);  R^T R d = g (float
+ * sweeps);  x += d (double); per column until ||d||_inf <= ||x||_inf *
+ * 2^-53 * sqrt(n), or the correction stops halving below ||x||_inf * 2^-40;
+ * a zero or non-finite R(i,i) of the float factor, a correction that does not
+ * contract, or 30 steps send the matrix to the double gels (A then holds its
+ * QR, X the first n rows of the solution; info as gels_compact_info). iter:
+ * refinement steps (max over the columns), -1 no single precision path for
+ * the shape (m x n not 16 x 8, 32 x 16 or 64 x 32, nrhs > 8, V < 4), -31 the
+ * double path ran. Shapes: trans 'N', n <= m <= 64,
+ * n + nrhs <= 64. */
+int oracle_dsgels_compact(char layout, char trans, int m, int n, int nrhs, double *ap, int ldap,
+                          const double *bp, int ldbp, double *xp, int ldxp, int *iter,
+                          int *info, int V, int nm)
+{
+    const bool col = opt(layout, 'C'), row = opt(layout, 'R');
+    if (!col && !row) return -1;
+    if (!opt(trans, 'N')) return -2;
+    if (m < 0 || m > 64) return -3;
+    if (n < 0 || n > m) return -4;
+    if (nrhs < 0 || n + nrhs > 64) return -5;
+    if (ldap < max1(row ? n : m)) return -7;
+    if (ldbp < max1(row ? nrhs : m)) return -9;
+    if (ldxp < max1(row ? nrhs : n)) return -11;
+    if (!vlen_ok(V)) return -14;
+    if (nm < 0) return -15;
+    for (int i = 0; i < nm; ++i) {
+        iter[i] = 0;
+        if (info) info[i] = 0;
+    }
+    if (n == 0 || nrhs == 0 || nm == 0) return 0;
+    const bool mixed = V >= 4 && nrhs <= 8 &&
+                       ((m == 16 && n == 8) || (m == 32 && n == 16) || (m == 64 && n == 32));
+    const size_t sa = group_stride(row, ldap, m, n, V), sb = group_stride(row, ldbp, m, nrhs, V),
+                 sx = group_stride(row, ldxp, n, nrhs, V);
+    const double cte = std::ldexp(1.0, -53) * std::sqrt((double)n), floor_ = std::ldexp(1.0, -40);
+    const int itermax = 30;
+    for_each_lane(nm, V, [&](long g, long v) {
+        const long idx = g * V + v;
+        if (idx >= nm) return;
+        const LV<double> A = make_view(ap + g * sa + v, row, ldap, V);
+        const LV<double> B = make_view(const_cast<double *>(bp) + g * sb + v, row, ldbp, V);
+        const LV<double> X = make_view(xp + g * sx + v, row, ldxp, V);
+        bool ok = mixed;
+        int steps = 0;
+        if (mixed) {
+            std::vector<float> F((size_t)m * n), tau(n);
+            const LV<float> Fv{F.data(), 1, m, 1};
+            for (long j = 0; j < n; ++j)
+                for (long i = 0; i < m; ++i)
+                    F[j * m + i] = (float)A(i, j);
+            geqrf_lane<float>(m, n, Fv, tau.data(), 1);
+            for (long i = 0; i < n && ok; ++i)
+                ok = Fv(i, i) != 0.f && std::isfinite(Fv(i, i));
+            for (long q = 0; q < nrhs && ok; ++q) {
+                std::vector<double> x(n, 0.0), gr(n);
+                std::vector<float> h(n);
+                double prev = INFINITY;
+                bool done = false;
+                for (int it = 0; it <= itermax; ++it) {
+                    for (long c = 0; c < n; ++c)
+                        gr[c] = 0.0;
+                    for (long i = 0; i < m; ++i) {
+                        double r = B(i, q);
+                        for (long c = 0; c < n; ++c)
+                            r = std::fma(-x[c], A(i, c), r);
+                        for (long c = 0; c < n; ++c)
+                            gr[c] = std::fma(A(i, c), r, gr[c]);
+                    }
+                    for (long c = 0; c < n; ++c)
+                        h[c] = (float)gr[c];
+                    for (long k = 0; k < n; ++k) {
+                        h[k] = h[k] / Fv(k, k);
+                        for (long i = k + 1; i < n; ++i)
+                            h[i] = std::fma(-Fv(k, i), h[k], h[i]);
+                    }
+                    for (long i = n - 1; i >= 0; --i) {
+                        float s = h[i];
+                        for (long k = i + 1; k < n; ++k)
+                            s = std::fma(-Fv(i, k), h[k], s);
+                        h[i] = s / Fv(i, i);
+                    }
+                    double dn = 0.0, xn = 0.0;
+                    for (long i = 0; i < n; ++i) {
+                        dn = std::fmax(dn, std::fabs((double)h[i]));
+                        x[i] = x[i] + (double)h[i];
+                    }
+                    for (long i = 0; i < n; ++i)
+                        xn = std::fmax(xn, std::fabs(x[i]));
+                    if (std::isnan(dn)) break;
+                    steps = it > steps ? it : steps;
+                    if (dn <= xn * cte || (it > 0 && dn > 0.5 * prev && dn <= xn * floor_)) {
+                        done = true;
+                        break;
+                    }
+                    if (!(dn < prev) && it > 0) break;
+                    prev = dn;
+                }
+                if (!done) {
+                    ok = false;
+                    break;
+                }
+                for (long i = 0; i < n; ++i)
+                    X(i, q) = x[i];
+            }
+        }
+        if (ok) {
+            iter[idx] = steps;
+            return;
+        }
+        iter[idx] = mixed ? -itermax - 1 : -1;
+        /* the double gels on one-slot copies of A and B (the trsm kernels take
+         * one slot stride for both operands), A's factor copied back */
+        std::vector<double> D((size_t)m * n), W((size_t)m * nrhs), tau(n);
+        const LV<double> Dv{D.data(), 1, m, 1}, Wv{W.data(), 1, m, 1};
+        for (long j = 0; j < n; ++j)
+            for (long i = 0; i < m; ++i)
+                D[j * m + i] = A(i, j);
+        for (long q = 0; q < nrhs; ++q)
+            for (long i = 0; i < m; ++i)
+                W[q * m + i] = B(i, q);
+        int bad = 0;
+        gels_lane<double>(true, m, n, nrhs, Dv, tau.data(), Wv, 1, info ? &bad : nullptr);
+        if (info) info[idx] = bad;
+        for (long j = 0; j < n; ++j)
+            for (long i = 0; i < m; ++i)
+                A(i, j) = D[j * m + i];
+        for (long q = 0; q < nrhs; ++q)
+            for (long i = 0; i < n; ++i)
+                X(i, q) = W[q * m + i];
+    });
+    return 0;
+}
+
 int oracle_dgels_batch(char trans, int nprob, const oracle_gels_problem *desc, double *ap,
                        double *bp, int *info)
 {

@@ -82,6 +82,14 @@ int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, i
                           const double *bp, int ldbp, double *xp, int ldxp, void *work,
                           size_t lwork, int *iter, int *info, int V, int nm);
 
+/* Mixed-precision least squares on the compact layout (no reference
+ * counterpart): single precision QR as the preconditioner of refinement on the
+ * normal equations with double residuals, double gels as the fallback; the
+ * library's dsgels_compact. Shapes: trans 'N', n <= m <= 64, n + nrhs <= 64. */
+int oracle_dsgels_compact(char layout, char trans, int m, int n, int nrhs, double *ap, int ldap,
+                          const double *bp, int ldbp, double *xp, int ldxp, int *iter,
+                          int *info, int V, int nm);
+
 /* One uneven batch (no reference counterpart): problem k is gels_compact on
  * the single column-major matrix A_k (m x n, lda) at ap + a_off and B_k
  * (max(m,n) x nrhs, ldb) at bp + b_off, i.e. the reference's

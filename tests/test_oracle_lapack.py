@@ -132,3 +132,64 @@ def test_dsposv_oracle_against_lapack():
             assert np.array_equal(F[k], np.where(np.tril(np.ones((n, n))) > 0, A[k], F[k]))
         eta = np.abs(B[k] - A[k] @ X[k]).max() / (np.abs(A[k]).sum(1).max() * np.abs(X[k]).max())
         assert eta <= 1e-15
+
+
+def _tall_cond(r, nm, m, n, logk):
+    """m x n matrices U diag(s) W^T with log10 of the singular values uniform on [0, logk]"""
+    U = np.linalg.qr(r.standard_normal((nm, m, n)))[0]
+    W = np.linalg.qr(r.standard_normal((nm, n, n)))[0]
+    s = 10.0 ** (-r.uniform(0, 1, (nm, n)) * logk)
+    s[:, 0] = 1.0
+    s[:, -1] = 10.0 ** -logk
+    return (U * s[:, None, :]) @ np.swapaxes(W, 1, 2)
+
+
+def test_dsgels_oracle_against_lapack():
+    """the mixed least squares: X against LAPACK dgels; refinement for kappa ~ 10,
+    the FP64 path (iter -31, A = its QR) for kappa ~ 1e6, iter -1 for a shape
+    without a single-precision kernel; info through the FP64 path"""
+    r = cbkt.rng(12)
+    for (m, n, nrhs, V) in ((64, 32, 2, 8), (16, 8, 3, 4), (24, 10, 2, 4)):
+        nm = 10
+        A = r.uniform(-1, 1, (nm, m, n))
+        A[[2, 7]] = _tall_cond(r, 2, m, n, 6)
+        B = r.standard_normal((nm, m, nrhs))
+        ap, bp = cbkt.pack(A, V), cbkt.pack(B, V, pad_diag=0.0)
+        bp0 = bp.copy()
+        xp = np.zeros(((nm + V - 1) // V) * n * nrhs * V)
+        it = np.zeros(nm, np.int32)
+        info = np.full(nm, -9, np.int32)
+        assert cbkt.oracle().call_raw("dsgels_compact", "C", "N", m, n, nrhs, ap, m, bp, m, xp, n,
+                                      it, info, V, nm) == 0
+        assert np.array_equal(bp, bp0)
+        X = cbkt.unpack(xp, nm, n, nrhs, V)
+        F = cbkt.unpack(ap, nm, m, n, V)
+        mixed = (m, n) in ((16, 8), (32, 16), (64, 32))
+        for k in range(nm):
+            _, x, inf = lapack.dgels(A[k], B[k])[:3]
+            assert inf == 0 and info[k] == 0
+            s = np.linalg.svd(A[k], compute_uv=False)
+            kap = s[0] / s[-1]
+            tol = 1e-14 * kap * (1 + kap * np.linalg.norm(B[k] - A[k] @ x[:n]) /
+                                 (s[0] * np.linalg.norm(x[:n]))) * np.abs(x[:n]).max() * 50
+            assert np.abs(X[k] - x[:n]).max() <= tol, (m, n, k)
+            if not mixed:
+                assert it[k] == -1
+            elif k in (2, 7):
+                assert it[k] == -31
+                qr = lapack.dgeqrf(A[k])[0]
+                assert np.allclose(np.abs(np.triu(F[k])), np.abs(np.triu(qr[:m])), atol=1e-12)
+            else:
+                assert 0 <= it[k] <= 10
+                assert np.array_equal(F[k], A[k])
+    # a zero column: FP32 R(j,j) = 0 -> FP64 path -> info j + 1
+    m, n, nrhs, V, nm = 32, 16, 1, 4, 4
+    A = r.uniform(-1, 1, (nm, m, n))
+    A[1, :, 5] = 0.0
+    ap, bp = cbkt.pack(A, V), cbkt.pack(r.standard_normal((nm, m, nrhs)), V, pad_diag=0.0)
+    xp = np.zeros(n * nrhs * V)
+    it = np.zeros(nm, np.int32)
+    info = np.zeros(nm, np.int32)
+    assert cbkt.oracle().call_raw("dsgels_compact", "C", "N", m, n, nrhs, ap, m, bp, m, xp, n,
+                                  it, info, V, nm) == 0
+    assert info.tolist() == [0, 6, 0, 0] and it[1] == -31 and np.all(it[[0, 2, 3]] >= 0)
