@@ -19,6 +19,10 @@
  * that runs ITERMAX steps is marked for the FP64 path (iter < 0), which a
  * second launch (the wavefront gels kernel on those matrices only) runs.
  *
+ * The right-hand sides go through the loop in blocks of NRB (2 or 4) in
+ * lockstep, so one pass over A per step serves the whole block; a column that
+ * has converged keeps its x while the others finish.
+ *
  * Layout in the team: lane t owns rows t + 16 r of A (as the QR team kernel);
  * after the QR only R's rows (r < Q/16) are kept, FP32. The sweeps work on a
  * right-hand side replicated in every lane of the team (g, then h, then d):
@@ -69,7 +73,7 @@ template <class T> __device__ __forceinline__ T team_sum(T x)
     return x;
 }
 
-template <int P, int Q>
+template <int P, int Q, int NRB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
 mixed_gels_kernel(MixedGelsArgs p)
 {
@@ -154,61 +158,77 @@ mixed_gels_kernel(MixedGelsArgs p)
 
     int steps = 0;
     const double cte = 0x1p-53 * __builtin_sqrt((double)Q), floor_ = 0x1p-40;
-    for (int q = 0; q < p.nrhs && ok; ++q) {
-        double x[RQ];
-        sfor<0, RQ>([&](auto r) { x[r] = 0.0; });
-        double prev = __builtin_inf();
-        bool done = false;
-        for (int it = 0; it <= p.itermax; ++it) {
+    /* NRB right-hand sides in lockstep: one pass over A per step serves them all */
+    for (int q0 = 0; q0 < p.nrhs && ok; q0 += NRB) {
+        double x[NRB][RQ], prev[NRB];
+        bool done[NRB];
+        unsigned bcol[NRB], xcol[NRB];
+        sfor<0, NRB>([&](auto j) {
+            sfor<0, RQ>([&](auto r) { x[j][r] = 0.0; });
+            prev[j] = __builtin_inf();
+            done[j] = q0 + j >= p.nrhs;
+            bcol[j] = done[j] ? kOOB : (q0 + j) * b_col;
+            xcol[j] = done[j] ? kOOB : (q0 + j) * x_col;
+        });
+        bool all = false;
+        for (int it = 0; it <= p.itermax && ok && !all; ++it) {
             /* g = A^T (b - A x), FP64, one row block of A at a time */
-            double gp[Q];
-            sfor<0, Q>([&](auto c) { gp[c] = 0.0; });
+            double gp[NRB][Q];
+            sfor<0, NRB>([&](auto j) { sfor<0, Q>([&](auto c) { gp[j][c] = 0.0; }); });
             sfor<0, R>([&](auto r) {
                 double arow[Q];
                 sfor<0, Q>([&](auto c) { arow[c] = bload(0.0, ra, a_row[r], c * a_col); });
-                double res = bload(0.0, rb, b_row[r], q * b_col);
-                sfor<0, Q>([&](auto c) { fnma_bc<c % TS>(res, x[c / TS], arow[c]); });
-                sfor<0, Q>([&](auto c) { gp[c] = __builtin_fma(arow[c], res, gp[c]); });
+                sfor<0, NRB>([&](auto j) {
+                    double res = bload(0.0, rb, b_row[r], bcol[j]);
+                    sfor<0, Q>([&](auto c) { fnma_bc<c % TS>(res, x[j][c / TS], arow[c]); });
+                    sfor<0, Q>([&](auto c) { gp[j][c] = __builtin_fma(arow[c], res, gp[j][c]); });
+                });
                 __builtin_amdgcn_sched_barrier(0);
             });
-            float h[Q];
-            sfor<0, Q>([&](auto c) { h[c] = (float)team_sum(gp[c]); });
-            /* R^T h = g */
-            sfor<0, Q>([&](auto kc) {
-                constexpr int k = kc;
-                const float dk = bcast<k % TS>(rr[k / TS][k]);
-                h[k] = h[k] / dk;
-                sfor<k + 1, Q>([&](auto i) { fnma_bc<k % TS>(h[i], rr[k / TS][i], h[k]); });
+            all = true;
+            sfor<0, NRB>([&](auto j) {
+                float h[Q];
+                sfor<0, Q>([&](auto c) { h[c] = (float)team_sum(gp[j][c]); });
+                /* R^T h = g */
+                sfor<0, Q>([&](auto kc) {
+                    constexpr int k = kc;
+                    const float dk = bcast<k % TS>(rr[k / TS][k]);
+                    h[k] = h[k] / dk;
+                    sfor<k + 1, Q>([&](auto i) { fnma_bc<k % TS>(h[i], rr[k / TS][i], h[k]); });
+                });
+                /* R d = h, d overwriting h from the bottom */
+                sfor_down<0, Q>([&](auto ic) {
+                    constexpr int i = ic;
+                    float s = h[i];
+                    sfor<i + 1, Q>([&](auto k) { s = __builtin_fmaf(-rr[i / TS][k], h[k], s); });
+                    s = s / rr[i / TS][i];
+                    h[i] = bcast<i % TS>(s);
+                });
+                if (!done[j]) {
+                    double dn = 0.0;
+                    sfor<0, Q>([&](auto i) {
+                        dn = fmax_nan(dn, (double)__builtin_fabsf(h[i]));
+                        x[j][i / TS] = x[j][i / TS] + ((t == i % TS) ? (double)h[i] : 0.0);
+                    });
+                    double xn = 0.0;
+                    sfor<0, RQ>([&](auto r) { xn = fmax_nan(xn, __builtin_fabs(x[j][r])); });
+                    xn = team_max(xn);
+                    steps = it > steps ? it : steps;
+                    if (dn <= xn * cte || (it > 0 && dn > 0.5 * prev[j] && dn <= xn * floor_))
+                        done[j] = true;
+                    else if ((!(dn < prev[j]) && it > 0) || it == p.itermax)
+                        ok = false; /* not contracting, or out of steps: FP64 path */
+                    prev[j] = dn;
+                }
+                all = all && done[j];
             });
-            /* R d = h, d overwriting h from the bottom */
-            sfor_down<0, Q>([&](auto ic) {
-                constexpr int i = ic;
-                float s = h[i];
-                sfor<i + 1, Q>([&](auto k) { s = __builtin_fmaf(-rr[i / TS][k], h[k], s); });
-                s = s / rr[i / TS][i];
-                h[i] = bcast<i % TS>(s);
-            });
-            double dn = 0.0;
-            sfor<0, Q>([&](auto i) {
-                dn = fmax_nan(dn, (double)__builtin_fabsf(h[i]));
-                x[i / TS] = x[i / TS] + ((t == i % TS) ? (double)h[i] : 0.0);
-            });
-            double xn = 0.0;
-            sfor<0, RQ>([&](auto r) { xn = fmax_nan(xn, __builtin_fabs(x[r])); });
-            xn = team_max(xn);
-            steps = it > steps ? it : steps;
-            if (dn <= xn * cte || (it > 0 && dn > 0.5 * prev && dn <= xn * floor_)) {
-                done = true;
-                break;
-            }
-            if (!(dn < prev) && it > 0) break; /* not contracting: FP64 path */
-            prev = dn;
         }
-        if (!done) {
-            ok = false;
-            break;
+        ok = ok && all;
+        if (ok) {
+            sfor<0, NRB>([&](auto j) {
+                sfor<0, RQ>([&](auto r) { bstore(x[j][r], rx, x_row[r], xcol[j]); });
+            });
         }
-        sfor<0, RQ>([&](auto r) { bstore(x[r], rx, x_row[r], q * x_col); });
     }
     if (mat < p.nm && t == 0) p.iter[mat] = ok ? steps : -p.itermax - 1;
 }
@@ -216,8 +236,12 @@ mixed_gels_kernel(MixedGelsArgs p)
 template <int P, int Q> hipError_t launch_pq(const MixedGelsArgs &args)
 {
     const unsigned blocks = (unsigned)((args.nmat + kTeamsPerBlock - 1) / kTeamsPerBlock);
-    hipLaunchKernelGGL((mixed_gels_kernel<P, Q>), dim3(xcd_grid(blocks)), dim3(64), 0,
-                       current_stream(), args);
+    if (args.nrhs <= 2)
+        hipLaunchKernelGGL((mixed_gels_kernel<P, Q, 2>), dim3(xcd_grid(blocks)), dim3(64), 0,
+                           current_stream(), args);
+    else
+        hipLaunchKernelGGL((mixed_gels_kernel<P, Q, 4>), dim3(xcd_grid(blocks)), dim3(64), 0,
+                           current_stream(), args);
     return hipGetLastError();
 }
 
