@@ -17,10 +17,20 @@ value = the headline path's matrices/s counting only matrices whose backward
 error max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) is
 <= 1e-14 (computed in FP64 on the GPU after the timed steps).
 
-Also reported: the roofline of the dominant kernel (algorithmic bytes per
-launch / its average duration from HIP events on the launch stream, against
-MI355X's 8 TB/s), and the reference's own CPU path (oracle/_ref, the
-reference's src/cbk.cpp compiled from its sources) timed on this host.
+Also reported, in the same JSON line:
+  dgels_n32    the least-squares half of the metric: dgels_compact (plain
+               FP64) against dsgels_compact (FP32 QR preconditioner, FP64
+               residuals) at m=64, n=32, nrhs=2, 131072 problems, counting
+               problems with ||A^T r||_2 / (||A||_F ||r||_2) <= 1e-14;
+  uneven_gels  dgels_batch / sgels_batch over 500000 least-squares problems
+               of uneven sizes (one recorded graph launch), against per-size
+               dgels_compact launches on a sample of the sizes;
+  roofline     of the dominant kernel (algorithmic bytes per launch / its
+               average duration from HIP events on the launch stream, against
+               MI355X's 8 TB/s; traffic from the rocprofv3 counters recorded in
+               profiles/);
+  cpu_baseline the reference's own CPU path (oracle/_ref, the reference's
+               src/cbk.cpp compiled from its sources) timed on this host.
 """
 
 import argparse
@@ -156,6 +166,75 @@ def cpu_baseline(A, B):
     return {"value": nm / best, "unit": "matrices/s", "cores": cores, "kind": kind,
             "sample": f"dposv_compact n=32 nrhs=4 FP64 V=8, {nm} matrices, best of 3, "
                       f"{os.path.basename(path) if path else 'oracle port'}; max backward error {eta:.1e}"}
+
+
+GELS_M, GELS_N, GELS_NRHS, GELS_NM = 64, 32, 2, 131072
+
+
+def gels_bench(torch, K, W, nm=GELS_NM):
+    """dgels_compact (plain FP64) against dsgels_compact (FP32 QR preconditioner,
+    FP64 residuals) on m=64, n=32, nrhs=2 least-squares problems, A and B
+    entries ~ U(-1,1), compact V=64. Counted: problems whose backward error
+    ||A^T r||_2 / (||A||_F ||r||_2) is <= 1e-14 (FP64, on the GPU)."""
+    import cbk_amd
+    m, n, q = GELS_M, GELS_N, GELS_NRHS
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2028)
+    Ad = torch.rand(nm, m, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    Bd = torch.rand(nm, m, q, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    A0, B0 = pack(torch, Ad, V), pack(torch, Bd, V)
+    T = torch.empty(((nm + V - 1) // V) * n * V, dtype=torch.float64, device="cuda")
+    X = torch.zeros(((nm + V - 1) // V), q, n, V, dtype=torch.float64, device="cuda")
+    it = torch.zeros(nm, dtype=torch.int32, device="cuda")
+    info = torch.zeros(nm, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def eta(Xs):
+        R = Bd - torch.matmul(Ad, Xs)
+        return (torch.matmul(Ad.transpose(1, 2).contiguous(), R).norm(dim=(1, 2)) /
+                (Ad.norm(dim=(1, 2)) * R.norm(dim=(1, 2))))
+
+    A, B = A0.clone(), B0.clone()
+    tp = []
+    for s in range(W + K):
+        A.copy_(A0)
+        B.copy_(B0)
+        ev[0].record(stream)
+        assert cbk_amd.call("d", "gels", "C", "N", m, n, q, A, m, B, m, T, V, nm) == 0
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        if s >= W:
+            tp.append(ev[0].elapsed_time(ev[1]) / 1e3)
+    e_plain = eta(unpack(torch, B, nm)[:, :n])
+    tm = []
+    for s in range(W + K):
+        A.copy_(A0)
+        ev[0].record(stream)
+        assert cbk_amd.call_ext("dsgels_compact", "C", "N", m, n, q, A, m, B0, m, X, n, it, info,
+                                V, nm) == 0
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        if s >= W:
+            tm.append(ev[0].elapsed_time(ev[1]) / 1e3)
+    e_mixed = eta(unpack(torch, X, nm))
+    t_p, t_m = sum(tp) / K, sum(tm) / K
+    ok_p, ok_m = int((e_plain <= ETA_MAX).sum().item()), int((e_mixed <= ETA_MAX).sum().item())
+    vals, cnts = torch.unique(it.cpu(), return_counts=True)
+    out = {"workload": f"dgels_compact / dsgels_compact m={m} n={n} nrhs={q}, {nm} problems, "
+                       "U(-1,1), compact V=64",
+           "plain_fp64": {"matrices_per_s": ok_p / t_p, "kernel_ms": t_p * 1e3, "ok_matrices": ok_p,
+                          "max_backward_error": e_plain.max().item()},
+           "mixed": {"matrices_per_s": ok_m / t_m, "kernel_ms": t_m * 1e3, "ok_matrices": ok_m,
+                     "max_backward_error": e_mixed.max().item(),
+                     "iter_histogram": {int(a): int(b) for a, b in zip(vals, cnts)}},
+           "backward_error": "||A^T r||_2 / (||A||_F ||r||_2) <= 1e-14"}
+    out["mixed_over_plain"] = out["mixed"]["matrices_per_s"] / out["plain_fp64"]["matrices_per_s"]
+    log(f"dgels n=32: plain {t_p * 1e3:.2f} ms, mixed {t_m * 1e3:.2f} ms, "
+        f"mixed/plain {out['mixed_over_plain']:.2f}, eta {e_plain.max().item():.1e} / {e_mixed.max().item():.1e}")
+    del Ad, Bd, A0, B0, A, B, T, X
+    torch.cuda.empty_cache()
+    return out
 
 
 K5_NPROB = 500000
@@ -451,6 +530,7 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
+    result["dgels_n32"] = gels_bench(torch, K=max(2, min(K, 10)), W=1)
     if not args.no_uneven:
         result["uneven_gels"] = uneven_bench(torch, K=max(2, min(K, 5)), W=1, nprob=args.uneven_nprob)
     if not args.no_cpu:
