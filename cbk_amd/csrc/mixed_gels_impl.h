@@ -62,7 +62,12 @@ struct MixedGelsArgs {
     int *iter;
 };
 
-constexpr int kTeamsPerBlock = 4;
+/* Teams per block. 16 (4 wavefronts): at V = 64 the block's 16 matrices fill
+ * whole 128-byte lines, so a line that A's re-reads go through serves one CU.
+ * At 64 x 32 a wavefront takes a whole SIMD's registers; one-wavefront blocks
+ * in the XCD-aware order (team_common.h) keep the 4 wavefronts of a line on
+ * one XCD's L2 without tying their schedules together. */
+template <int P> constexpr int teams_per_block() { return P >= 64 ? 4 : 16; }
 
 template <class T> __device__ __forceinline__ T team_sum(T x)
 {
@@ -74,7 +79,8 @@ template <class T> __device__ __forceinline__ T team_sum(T x)
 }
 
 template <int P, int Q, int NRB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
+__global__ __launch_bounds__(teams_per_block<P>() * TS)
+__attribute__((amdgpu_waves_per_eu(P >= 64 || NRB > 2 ? 1 : 2, P >= 64 || NRB > 2 ? 1 : 8))) void
 mixed_gels_kernel(MixedGelsArgs p)
 {
     constexpr int R = P / TS;  /* row blocks of A */
@@ -82,7 +88,8 @@ mixed_gels_kernel(MixedGelsArgs p)
     constexpr unsigned SZ = 8;
     const int t = threadIdx.x & (TS - 1);
     const int team = threadIdx.x / TS;
-    const long mat0 = (long)xcd_block(blockIdx.x) * kTeamsPerBlock + 4 * wave_in_block();
+    const long mat0 = (long)(P >= 64 ? xcd_block(blockIdx.x) : blockIdx.x) * teams_per_block<P>() +
+                      4 * wave_in_block();
     if (mat0 >= p.nmat) return;
     const long mat = mat0 + (team & 3);
     const long g = mat0 / p.V;
@@ -105,13 +112,32 @@ mixed_gels_kernel(MixedGelsArgs p)
         x_row[r] = (i < Q) ? ((unsigned)(i * p.x_si) * V + v) * SZ : kOOB;
     });
 
-    /* ---- FP32 QR of A (geqr2, qr_team_impl.h), R only ---- */
+    /* ---- FP32 QR of A (geqr2, qr_team_impl.h), R only; the load of A also
+     * forms the first step's g = A^T b of the first block of right-hand
+     * sides (x_0 = 0), so that step needs no pass of its own over A ---- */
     float rr[RQ][Q];
     bool ok = true;
+    /* (two right-hand sides at most: with four, the extra registers cost more
+     * than the pass they save) */
+    constexpr bool FOLD = NRB <= 2;
+    double gp0[FOLD ? NRB : 1][Q];
     {
         float f[R][Q];
+        sfor<0, FOLD ? NRB : 1>([&](auto j) { sfor<0, Q>([&](auto c) { gp0[j][c] = 0.0; }); });
         sfor<0, R>([&](auto r) {
-            sfor<0, Q>([&](auto c) { f[r][c] = (float)bload(0.0, ra, a_row[r], c * a_col); });
+            double bv[NRB];
+            if constexpr (FOLD) {
+                sfor<0, NRB>([&](auto j) {
+                    bv[j] = bload(0.0, rb, b_row[r], j < p.nrhs ? j * b_col : kOOB);
+                });
+            }
+            sfor<0, Q>([&](auto c) {
+                const double a = bload(0.0, ra, a_row[r], c * a_col);
+                f[r][c] = (float)a;
+                if constexpr (FOLD) {
+                    sfor<0, NRB>([&](auto j) { gp0[j][c] = __builtin_fma(a, bv[j], gp0[j][c]); });
+                }
+            });
         });
         sfor<0, Q>([&](auto kc) {
             constexpr int kk = kc;
@@ -174,8 +200,11 @@ mixed_gels_kernel(MixedGelsArgs p)
         for (int it = 0; it <= p.itermax && ok && !all; ++it) {
             /* g = A^T (b - A x), FP64, one row block of A at a time */
             double gp[NRB][Q];
-            sfor<0, NRB>([&](auto j) { sfor<0, Q>([&](auto c) { gp[j][c] = 0.0; }); });
-            sfor<0, R>([&](auto r) {
+            const bool first = FOLD && it == 0 && q0 == 0;
+            sfor<0, NRB>([&](auto j) {
+                sfor<0, Q>([&](auto c) { gp[j][c] = first ? gp0[FOLD ? j : 0][c] : 0.0; });
+            });
+            if (!first) sfor<0, R>([&](auto r) {
                 double arow[Q];
                 sfor<0, Q>([&](auto c) { arow[c] = bload(0.0, ra, a_row[r], c * a_col); });
                 sfor<0, NRB>([&](auto j) {
@@ -235,12 +264,14 @@ mixed_gels_kernel(MixedGelsArgs p)
 
 template <int P, int Q> hipError_t launch_pq(const MixedGelsArgs &args)
 {
-    const unsigned blocks = (unsigned)((args.nmat + kTeamsPerBlock - 1) / kTeamsPerBlock);
+    constexpr int TPB = teams_per_block<P>();
+    unsigned blocks = (unsigned)((args.nmat + TPB - 1) / TPB);
+    if (P >= 64) blocks = xcd_grid(blocks);
     if (args.nrhs <= 2)
-        hipLaunchKernelGGL((mixed_gels_kernel<P, Q, 2>), dim3(xcd_grid(blocks)), dim3(64), 0,
+        hipLaunchKernelGGL((mixed_gels_kernel<P, Q, 2>), dim3(blocks), dim3(TPB * TS), 0,
                            current_stream(), args);
     else
-        hipLaunchKernelGGL((mixed_gels_kernel<P, Q, 4>), dim3(xcd_grid(blocks)), dim3(64), 0,
+        hipLaunchKernelGGL((mixed_gels_kernel<P, Q, 4>), dim3(blocks), dim3(TPB * TS), 0,
                            current_stream(), args);
     return hipGetLastError();
 }
