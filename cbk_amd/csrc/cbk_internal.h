@@ -72,6 +72,12 @@ template <class T>
 hipError_t launch_sym_team(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
                            Operand<T> B, int V, int nm, int *info);
 
+/* trsm, side L, forward sweep (op(A) lower), m in {16, 24, 32} */
+bool trsm_team_supports(bool left, bool up, bool tran, int m);
+template <class T>
+hipError_t launch_trsm_team(bool up, bool tran, bool unit, int m, int n, T alpha, Operand<T> A,
+                            Operand<T> B, int V, int nm);
+
 /* orders and right-hand-side counts served by the fused mixed posv team kernel */
 bool mixed_team_supports(int n, int nrhs);
 
