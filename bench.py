@@ -22,6 +22,8 @@ Also reported, in the same JSON line:
                FP64) against dsgels_compact (FP32 QR preconditioner, FP64
                residuals) at m=64, n=32, nrhs=2, 131072 problems, counting
                problems with ||A^T r||_2 / (||A||_F ||r||_2) <= 1e-14;
+  configs      the other configurations (K1 dposv n=8, K2 dgels/dgeqrf/dormqr
+               16x8, K3 sysvnp and trsm 24x24), one call each;
   uneven_gels  dgels_batch / sgels_batch over 500000 least-squares problems
                of uneven sizes (one recorded graph launch), against per-size
                dgels_compact launches on a sample of the sizes;
@@ -56,6 +58,7 @@ def parse():
     ap.add_argument("--nm", type=int, default=NM)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch gels leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the K1-K3 configuration leg")
     ap.add_argument("--uneven-nprob", type=int, default=K5_NPROB)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (the known-size kernel "
@@ -233,6 +236,85 @@ def gels_bench(torch, K, W, nm=GELS_NM):
     log(f"dgels n=32: plain {t_p * 1e3:.2f} ms, mixed {t_m * 1e3:.2f} ms, "
         f"mixed/plain {out['mixed_over_plain']:.2f}, eta {e_plain.max().item():.1e} / {e_mixed.max().item():.1e}")
     del Ad, Bd, A0, B0, A, B, T, X
+    torch.cuda.empty_cache()
+    return out
+
+
+def configs_bench(torch):
+    """the other configurations of the hot path, each timed as one call over
+    inputs resident in HBM (best of 3 after a warm-up; inputs restored
+    outside the timed region), as matrices/s and algorithmic GB/s:
+      K1  dposv n=8 nrhs=1, 2^21 matrices (the CPU config's shape, V=64 on the GPU)
+      K2  dgels 16x8 nrhs=2, 2^20; dgeqrf and dormqr (L, T) of the same shape
+      K3  dsysvnp / ssysvnp n=24 nrhs=24 (off-diagonals N(0,1), diagonal +-26),
+          dtrsm / strsm L L N U 24x24 with 24 right-hand sides, 262144 matrices"""
+    import cbk_amd
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2029)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    stream = torch.cuda.current_stream()
+    out = {}
+
+    def timeit(f, setup):
+        ts = []
+        for i in range(4):
+            setup()
+            torch.cuda.synchronize()
+            ev[0].record(stream)
+            assert f() == 0
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            if i:
+                ts.append(ev[0].elapsed_time(ev[1]) / 1e3)
+        return min(ts)
+
+    def rec(key, t, nm, bpm, what):
+        out[key] = {"what": what, "us": t * 1e6, "matrices_per_s": nm / t,
+                    "algorithmic_gbs": bpm * nm / t / 1e9, "frac_of_8tbs": bpm * nm / t / 8e12}
+        log(f"{key}: {t * 1e6:.1f} us, {nm / t:.3e} matrices/s, {bpm * nm / t / 1e9:.0f} GB/s")
+
+    nm, n = 1 << 21, 8
+    G = torch.randn(nm, n, n, dtype=torch.float64, device="cuda", generator=g)
+    A0 = pack(torch, G @ G.transpose(1, 2) + 8 * torch.eye(n, dtype=torch.float64, device="cuda"), V)
+    del G
+    B0 = pack(torch, torch.randn(nm, n, 1, dtype=torch.float64, device="cuda", generator=g), V)
+    A, B = A0.clone(), B0.clone()
+    t = timeit(lambda: cbk_amd.call("d", "posv", "C", "L", n, 1, A, n, B, n, V, nm),
+               lambda: (A.copy_(A0), B.copy_(B0)))
+    rec("K1_dposv_n8", t, nm, 8 * (n * (n + 1) + 2 * n), "dposv_compact n=8 nrhs=1, A=GG^T+8I, 2^21, V=64")
+    del A0, B0, A, B
+
+    nm, m, n, q = 1 << 20, 16, 8, 2
+    A0 = pack(torch, torch.rand(nm, m, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1, V)
+    B0 = pack(torch, torch.rand(nm, m, q, dtype=torch.float64, device="cuda", generator=g) * 2 - 1, V)
+    T = torch.empty(nm * n, dtype=torch.float64, device="cuda")
+    A, B = A0.clone(), B0.clone()
+    t = timeit(lambda: cbk_amd.call("d", "gels", "C", "N", m, n, q, A, m, B, m, T, V, nm),
+               lambda: (A.copy_(A0), B.copy_(B0)))
+    rec("K2_dgels_16x8", t, nm, 8 * (2 * m * n + n + 2 * m * q), "dgels_compact 16x8 nrhs=2, U(-1,1), 2^20")
+    t = timeit(lambda: cbk_amd.call("d", "geqrf", "C", m, n, A, m, T, V, nm), lambda: A.copy_(A0))
+    rec("K2_dgeqrf_16x8", t, nm, 8 * (2 * m * n + n), "dgeqrf_compact 16x8")
+    t = timeit(lambda: cbk_amd.call("d", "ormqr", "C", "L", "T", m, q, n, A, m, T, B, m, V, nm),
+               lambda: B.copy_(B0))
+    rec("K2_dormqr_16x2_k8", t, nm, 8 * (m * n + n + 2 * m * q), "dormqr_compact L T, C 16x2, k=8")
+    del A0, B0, A, B, T
+
+    nm, n, q = 262144, 24, 24
+    for p, dt, es in (("d", torch.float64, 8), ("s", torch.float32, 4)):
+        M = torch.tril(torch.randn(nm, n, n, dtype=torch.float64, device="cuda", generator=g), -1)
+        sign = torch.where(torch.rand(nm, n, device="cuda", generator=g) < 0.5, -26.0, 26.0).double()
+        M = M + M.transpose(1, 2) + torch.diag_embed(sign)
+        A0 = pack(torch, M.to(dt), V)
+        del M
+        B0 = pack(torch, torch.randn(nm, n, q, dtype=torch.float64, device="cuda", generator=g).to(dt), V)
+        A, B = A0.clone(), B0.clone()
+        t = timeit(lambda: cbk_amd.call(p, "sysvnp", "C", "L", n, q, A, n, B, n, V, nm),
+                   lambda: (A.copy_(A0), B.copy_(B0)))
+        rec(f"K3_{p}sysvnp_n24", t, nm, es * (n * (n + 1) + 2 * n * q), f"{p}sysvnp_compact n=24 nrhs=24")
+        t = timeit(lambda: cbk_amd.call(p, "trsm", "C", "L", "L", "N", "U", n, q, 1.0, A0, n, B, n, V, nm),
+                   lambda: B.copy_(B0))
+        rec(f"K3_{p}trsm_24x24", t, nm, es * (n * (n - 1) // 2 + 2 * n * q), f"{p}trsm_compact L L N U 24x24 nrhs=24")
+        del A0, B0, A, B
     torch.cuda.empty_cache()
     return out
 
@@ -531,6 +613,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
     result["dgels_n32"] = gels_bench(torch, K=max(2, min(K, 10)), W=1)
+    if not args.no_configs:
+        result["configs"] = configs_bench(torch)
     if not args.no_uneven:
         result["uneven_gels"] = uneven_bench(torch, K=max(2, min(K, 5)), W=1, nprob=args.uneven_nprob)
     if not args.no_cpu:

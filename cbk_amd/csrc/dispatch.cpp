@@ -12,6 +12,8 @@ hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand
 {
     if (current_policy() == CBK_POLICY_AUTO && sym_team_supports(n) && V >= 4)
         return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+    if (current_policy() != CBK_POLICY_LANE && reg_chol_supports(n))
+        return launch_reg_chol<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
 }
 
