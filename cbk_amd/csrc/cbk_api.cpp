@@ -90,9 +90,11 @@ int ormqr(char layout, char side, char trans, int m, int n, int k, const T *ap, 
     if (m == 0 || n == 0 || k == 0 || nm == 0) return 0;
     /* A is the (ldap, k) reflector batch: group stride ldap*k*V column-major
      * (src/cbk_ormqr_compact.hpp:144) */
-    return status(launch_ormqr_lane<T>(left, opt(trans, 'T'), m, n, k,
-                                       operand(const_cast<T *>(ap), ldap, row, nq, k, V), taup,
-                                       operand(cp, ldcp, row, m, n, V), V, nm));
+    const Operand<T> A = operand(const_cast<T *>(ap), ldap, row, nq, k, V);
+    const Operand<T> C = operand(cp, ldcp, row, m, n, V);
+    if (current_policy() != CBK_POLICY_LANE && V >= 4 && ormqr_team_supports(nq))
+        return status(launch_ormqr_team<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
+    return status(launch_ormqr_lane<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
 }
 
 /* src/cbk.cpp:107-127 */

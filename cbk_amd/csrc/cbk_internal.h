@@ -85,6 +85,12 @@ template <class T>
 hipError_t launch_trsm_team(bool up, bool tran, bool unit, int m, int n, T alpha, Operand<T> A,
                             Operand<T> B, int V, int nm);
 
+/* ormqr with reflector length nq (m for side L, n for side R) in {16, 32} */
+bool ormqr_team_supports(int nq);
+template <class T>
+hipError_t launch_ormqr_team(bool left, bool tran, int m, int n, int k, Operand<T> A,
+                             const T *tau, Operand<T> C, int V, int nm);
+
 /* orders and right-hand-side counts served by the fused mixed posv team kernel */
 bool mixed_team_supports(int n, int nrhs);
 
