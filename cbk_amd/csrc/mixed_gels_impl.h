@@ -176,6 +176,7 @@ mixed_gels_kernel(MixedGelsArgs p)
             const int i = t + TS * r;
             sfor<0, Q>([&](auto c) { rr[r][c] = (c >= i) ? f[r][c] : 0.f; });
         });
+        sfor<0, RQ>([&](auto r) { dpp_ready_from<0>(rr[r]); }); /* the sweeps' DPP sources */
         sfor<0, Q>([&](auto kc) {
             const float d = bcast<kc % TS>(rr[kc / TS][kc]);
             ok = ok && (d != 0.f) && __builtin_isfinite(d);
@@ -204,12 +205,13 @@ mixed_gels_kernel(MixedGelsArgs p)
             sfor<0, NRB>([&](auto j) {
                 sfor<0, Q>([&](auto c) { gp[j][c] = first ? gp0[FOLD ? j : 0][c] : 0.0; });
             });
+            sfor<0, NRB>([&](auto j) { dpp_ready_from<0>(x[j]); });
             if (!first) sfor<0, R>([&](auto r) {
                 double arow[Q];
                 sfor<0, Q>([&](auto c) { arow[c] = bload(0.0, ra, a_row[r], c * a_col); });
                 sfor<0, NRB>([&](auto j) {
                     double res = bload(0.0, rb, b_row[r], bcol[j]);
-                    sfor<0, Q>([&](auto c) { fnma_bc<c % TS>(res, x[j][c / TS], arow[c]); });
+                    sfor<0, Q>([&](auto c) { fnma_bcr<c % TS>(res, x[j][c / TS], arow[c]); });
                     sfor<0, Q>([&](auto c) { gp[j][c] = __builtin_fma(arow[c], res, gp[j][c]); });
                 });
                 __builtin_amdgcn_sched_barrier(0);
@@ -223,7 +225,7 @@ mixed_gels_kernel(MixedGelsArgs p)
                     constexpr int k = kc;
                     const float dk = bcast<k % TS>(rr[k / TS][k]);
                     h[k] = h[k] / dk;
-                    sfor<k + 1, Q>([&](auto i) { fnma_bc<k % TS>(h[i], rr[k / TS][i], h[k]); });
+                    sfor<k + 1, Q>([&](auto i) { fnma_bcr<k % TS>(h[i], rr[k / TS][i], h[k]); });
                 });
                 /* R d = h, d overwriting h from the bottom */
                 sfor_down<0, Q>([&](auto ic) {

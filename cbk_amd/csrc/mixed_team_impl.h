@@ -16,8 +16,11 @@
  * dsposv's ITER does.
  *
  * The FP32 factor (chol_factor_fast: hardware reciprocal square root) and the
- * FP32 solves are the team building blocks of team_chol.h; the symmetric
- * residual sums A(i,c) x_c over c ascending with one FP64 fma each.
+ * FP32 solves are the team building blocks of team_chol.h; the solves read the
+ * factor from the team's LDS (chol_solve_split), so that during refinement the
+ * registers hold the FP64 A, B, X and the residual only, and B is loaded after
+ * the factor: two wavefronts fit a SIMD at n = 32, four at n = 16. The
+ * symmetric residual sums A(i,c) x_c over c ascending with one FP64 fma each.
  */
 #ifndef CBK_MIXED_TEAM_IMPL_H
 #define CBK_MIXED_TEAM_IMPL_H
@@ -49,14 +52,14 @@ struct MixedArgs {
     int *info;
 };
 
-/* 4 teams (one wavefront) per block: at one wavefront per SIMD a 4-wave
- * block would hold its CU until its slowest matrix converged */
+/* 4 teams (one wavefront) per block: with few wavefronts per SIMD a 4-wave
+ * block would hold its CU's slots until its slowest matrix converged */
 constexpr int kTeamsPerBlock = 4;
 
 template <int N> hipError_t launch_n(const MixedArgs &args);
 
 template <int N, int NRB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N <= 16 ? 4 : 2, N <= 16 ? 4 : 2))) void
 mixed_posv_kernel(MixedArgs p)
 {
     constexpr int R = rows_per_lane<N>();
@@ -104,12 +107,7 @@ mixed_posv_kernel(MixedArgs p)
             a64[r][c] = bload(0.0, ra, off, 0);
         });
     });
-    double b64[R][NRB];
-    sfor<0, R>([&](auto r) {
-        sfor<0, NRB>([&](auto q) { b64[r][q] = bload(0.0, rb, b_row[r], bcol[q]); });
-    });
-
-    /* ||A||_inf (dlansy 'I') and the FP32 overflow test (dlat2s / dlag2s) */
+    /* ||A||_inf (dlansy 'I') and the FP32 overflow test of A (dlat2s) */
     double rowsum[R];
     bool over = false;
     sfor<0, R>([&](auto r) {
@@ -118,7 +116,6 @@ mixed_posv_kernel(MixedArgs p)
             rowsum[r] += __builtin_fabs(a64[r][c]);
             over |= __builtin_fabs(a64[r][c]) > 3.4028234663852886e38;
         });
-        sfor<0, NRB>([&](auto q) { over |= __builtin_fabs(b64[r][q]) > 3.4028234663852886e38; });
     });
     double anrm = 0.0;
     sfor<0, R>([&](auto r) {
@@ -127,14 +124,15 @@ mixed_posv_kernel(MixedArgs p)
     anrm = team_max(anrm);
     const double cte = anrm * 0x1p-53 * __builtin_sqrt((double)N);
     /* any lane of the team overflowing sends the whole matrix to FP64 */
-    const bool team_over = team_max(over ? 1.0 : 0.0) > 0.0;
+    const bool a_over = team_max(over ? 1.0 : 0.0) > 0.0;
 
     int iter = 0;
     bool done = false;
+    bool ready = false; /* the FP32 factor is in LDS */
     float *L32 = reinterpret_cast<float *>(lds + team * chol_lds<N>());
-    double x64[R][NRB];
-    if (!team_over) {
-        float a32[R][N], dg32[R], dinv32[R];
+    float dinv32[R];
+    if (!a_over) {
+        float a32[R][N], dg32[R];
         sfor<0, R>([&](auto r) {
             sfor<0, cmin(TS * (r + 1), N)>([&](auto c) { a32[r][c] = (float)a64[r][c]; });
         });
@@ -143,18 +141,44 @@ mixed_posv_kernel(MixedArgs p)
             iter = -3;
         }
         else {
-            chol_prepare<float, N, false>(a32, dg32, dinv32, L32, t);
+            chol_prepare_split<N>(a32, dg32, dinv32, L32, t);
+            ready = true;
+        }
+    }
+    else {
+        iter = -2;
+    }
+    /* B is read after the factor, whose registers it would otherwise share;
+     * an overflowing B (dlag2s) sends the matrix to FP64 like an overflowing
+     * A, whatever the factor did (dsposv tests B first) */
+    double b64[R][NRB];
+    sfor<0, R>([&](auto r) {
+        sfor<0, NRB>([&](auto q) { b64[r][q] = bload(0.0, rb, b_row[r], bcol[q]); });
+    });
+    bool b_over = false;
+    sfor<0, R>([&](auto r) {
+        sfor<0, NRB>([&](auto q) { b_over |= __builtin_fabs(b64[r][q]) > 3.4028234663852886e38; });
+    });
+    if (team_max(b_over ? 1.0 : 0.0) > 0.0) {
+        iter = -2;
+        ready = false;
+    }
+
+    double x64[R][NRB];
+    if (ready) {
+        {
             float w[R][NRB];
             sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { w[r][q] = (float)b64[r][q]; }); });
-            chol_solve<float, N, NRB, false>(a32, dinv32, w, L32, t);
+            chol_solve_split<N, NRB>(dinv32, w, L32, t);
             sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { x64[r][q] = (double)w[r][q]; }); });
             for (int it = 0;; ++it) {
                 /* r = b - A x, FP64 */
                 double res[R][NRB];
                 sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { res[r][q] = b64[r][q]; }); });
+                sfor<0, R>([&](auto r) { dpp_ready_from<0>(x64[r]); });
                 sfor<0, N>([&](auto c) {
                     sfor<0, NRB>([&](auto q) {
-                        sfor<0, R>([&](auto r) { fnma_bc<c % TS>(res[r][q], x64[c / TS][q], a64[r][c]); });
+                        sfor<0, R>([&](auto r) { fnma_bcr<c % TS>(res[r][q], x64[c / TS][q], a64[r][c]); });
                     });
                 });
                 /* every column: ||r||_inf <= ||x||_inf * cte (NaN fails) */
@@ -191,15 +215,12 @@ mixed_posv_kernel(MixedArgs p)
                     iter = -2;
                     break;
                 }
-                chol_solve<float, N, NRB, false>(a32, dinv32, w, L32, t);
+                chol_solve_split<N, NRB>(dinv32, w, L32, t);
                 sfor<0, R>([&](auto r) {
                     sfor<0, NRB>([&](auto q) { x64[r][q] = x64[r][q] + (double)w[r][q]; });
                 });
             }
         }
-    }
-    else {
-        iter = -2;
     }
 
     int dinfo = 0;

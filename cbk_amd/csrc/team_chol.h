@@ -80,7 +80,7 @@ __device__ __forceinline__ int chol_factor(T (&a)[rows_per_lane<N>()][N],
         auto panel = [&](auto c0, auto c1) {
             sfor<c0, c1>([&](auto c) {
                 sfor<0, R>([&](auto r) {
-                    if constexpr (c < TS * (r + 1)) fnma_bc<c % TS>(a[r][c], wv[c / TS], a[r][j]);
+                    if constexpr (c < TS * (r + 1)) fnma_bcr<c % TS>(a[r][c], wv[c / TS], a[r][j]);
                 });
             });
         };
@@ -90,7 +90,7 @@ __device__ __forceinline__ int chol_factor(T (&a)[rows_per_lane<N>()][N],
             sfor<c0, c1>([&](auto c) {
                 sfor<0, R>([&](auto r) {
                     if constexpr (c < TS * (r + 1))
-                        fnma_bc<c % TS>(a[r][c], LDL ? wd[c / TS] : a[c / TS][j], a[r][j]);
+                        fnma_bcr<c % TS>(a[r][c], LDL ? wd[c / TS] : a[c / TS][j], a[r][j]);
                 });
             });
         };
@@ -103,6 +103,7 @@ __device__ __forceinline__ int chol_factor(T (&a)[rows_per_lane<N>()][N],
             sfor<0, R>([&](auto r) {
                 if constexpr (j < TS * (r + 1)) wv[r] = a[r][j] * inv;
             });
+            dpp_ready_from<rj>(wv);
             panel(I1{}, I2{});
             nxt = load_pivot<T, N, LDL, j + 1>(a);
             panel(I2{}, IPE{});
@@ -125,6 +126,10 @@ __device__ __forceinline__ int chol_factor(T (&a)[rows_per_lane<N>()][N],
                 sfor<0, R>([&](auto r) {
                     if constexpr (j < TS * (r + 1)) wd[r] = a[r][j] * piv;
                 });
+                dpp_ready_from<rj>(wd);
+            }
+            else {
+                dpp_ready_col<rj, j>(a);
             }
             if constexpr (j + 1 == pe) {
                 trailing(IPE{}, IPE1{});
@@ -164,10 +169,11 @@ __device__ __forceinline__ int chol_factor_fast(float (&a)[rows_per_lane<N>()][N
         sfor<0, R>([&](auto r) {
             if constexpr (j < TS * (r + 1)) a[r][j] = a[r][j] * rs;
         });
+        if constexpr (j + 1 < N) dpp_ready_col<rj, j>(a);
         float npiv = piv, nrs = rs;
         sfor<j + 1, N>([&](auto c) {
             sfor<0, R>([&](auto r) {
-                if constexpr (c < TS * (r + 1)) fnma_bc<c % TS>(a[r][c], a[c / TS][j], a[r][j]);
+                if constexpr (c < TS * (r + 1)) fnma_bcr<c % TS>(a[r][c], a[c / TS][j], a[r][j]);
             });
             if constexpr (c == j + 1) {
                 npiv = bcast<c % TS>(a[c / TS][c]);
@@ -231,8 +237,12 @@ __device__ __forceinline__ void chol_solve(const T (&a)[rows_per_lane<N>()][N],
     sfor<0, N>([&](auto ic) {
         constexpr int i = ic;
         constexpr int ri = i / TS, ti = i % TS;
+        /* the pivot row's own update last: no DPP read may follow the write
+         * of its source within the step */
+        dpp_ready_from<0>(b[ri]);
         sfor<0, NRB>([&](auto q) {
-            sfor<ri, R>([&](auto r) { fnma_bc<ti>(b[r][q], b[ri][q], a[r][i]); });
+            sfor<ri + 1, R>([&](auto r) { fnma_bcr<ti>(b[r][q], b[ri][q], a[r][i]); });
+            fnma_bcr<ti>(b[ri][q], b[ri][q], a[ri][i]);
         });
         __builtin_amdgcn_sched_barrier(0);
     });
@@ -256,8 +266,10 @@ __device__ __forceinline__ void chol_solve(const T (&a)[rows_per_lane<N>()][N],
         constexpr int rl = l / TS, tl = l % TS;
         T lvn[R];
         if constexpr (l > 0) load_row(std::integral_constant<int, l - 1>{}, lvn);
+        dpp_ready_from<0>(b[rl]);
         sfor<0, NRB>([&](auto q) {
-            sfor<0, rl + 1>([&](auto r) { fnma_bc<tl>(b[r][q], b[rl][q], lv[r]); });
+            sfor<0, rl>([&](auto r) { fnma_bcr<tl>(b[r][q], b[rl][q], lv[r]); });
+            fnma_bcr<tl>(b[rl][q], b[rl][q], lv[rl]);
         });
         if constexpr (l > 0) {
             sfor<0, R>([&](auto r) { lv[r] = lvn[r]; });
@@ -269,6 +281,108 @@ __device__ __forceinline__ void chol_solve(const T (&a)[rows_per_lane<N>()][N],
             sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * dinv[r]; });
         });
     }
+}
+
+/* The FP32 Cholesky solve with both sweeps reading L from LDS, so the
+ * factor's registers are free once it is prepared (the mixed kernel keeps
+ * the FP64 A in registers for its residuals). The team's region holds two
+ * FP32 copies: at L the packed rows of chol_prepare (L(i,c)/L(i,i) at
+ * i(i+1)/2 + c, for the backward sweep), at L + chol_lds<N>() the packed
+ * columns scaled by 1/L(c,c) (L(i,c) at cN - c(c-1)/2 + i - c, for the
+ * forward sweep: lanes read consecutive words). Both end in a 0.0 slot. */
+template <int N> constexpr int fwd_at(int i, int c) { return c * N - c * (c - 1) / 2 + (i - c); }
+
+template <int N>
+__device__ __forceinline__ void chol_prepare_split(const float (&a)[rows_per_lane<N>()][N],
+                                                   const float (&dg)[rows_per_lane<N>()],
+                                                   float (&dinv)[rows_per_lane<N>()], float *L,
+                                                   int t)
+{
+    constexpr int R = rows_per_lane<N>();
+    constexpr int ZERO = N * (N + 1) / 2, SINK = ZERO + 1;
+    float *Lc = L + chol_lds<N>();
+    sfor<0, R>([&](auto r) { dinv[r] = 1.0f / dg[r]; });
+    sfor<0, N>([&](auto c) {
+        const float dc = bcast<c % TS>(dinv[c / TS]);
+        sfor<c / TS, R>([&](auto r) {
+            const int i = t + TS * r;
+            const bool below = (i < N) && (c < TS * r || c < i);
+            L[below ? i * (i + 1) / 2 + c : SINK] = a[r][c] * dinv[r];
+            Lc[below ? fwd_at<N>(i, c) : SINK] = a[r][c] * dc;
+        });
+    });
+    if (t == 0) {
+        L[ZERO] = 0.0f;
+        Lc[ZERO] = 0.0f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int N, int NRB>
+__device__ __forceinline__ void chol_solve_split(const float (&dinv)[rows_per_lane<N>()],
+                                                 float (&b)[rows_per_lane<N>()][NRB],
+                                                 const float *L, int t)
+{
+    constexpr int R = rows_per_lane<N>();
+    constexpr int ZERO = N * (N + 1) / 2;
+    constexpr int D = 4; /* forward coefficients are read D steps ahead */
+    const float *Lc = L + chol_lds<N>();
+    /* forward: step i needs L(t+16r, i)/L(i,i) for the rows below i */
+    auto load_col = [&](auto ic, float(&cv)[R]) {
+        constexpr int i = ic;
+        sfor<i / TS, R>([&](auto r) {
+            const int row = t + TS * r;
+            cv[r] = Lc[(row > i && row < N) ? fwd_at<N>(row, i) : ZERO];
+        });
+    };
+    float ring[D][R];
+    sfor<0, cmin(D, N)>([&](auto ic) { load_col(ic, ring[ic]); });
+    sfor<0, N>([&](auto ic) {
+        constexpr int i = ic;
+        constexpr int ri = i / TS, ti = i % TS;
+        float cv[R];
+        sfor<ri, R>([&](auto r) { cv[r] = ring[i % D][r]; });
+        if constexpr (i + D < N) load_col(std::integral_constant<int, i + D>{}, ring[i % D]);
+        dpp_ready_from<0>(b[ri]);
+        sfor<0, NRB>([&](auto q) {
+            sfor<ri + 1, R>([&](auto r) { fnma_bcr<ti>(b[r][q], b[ri][q], cv[r]); });
+            fnma_bcr<ti>(b[ri][q], b[ri][q], cv[ri]);
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    sfor<0, R>([&](auto r) {
+        sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * dinv[r]; });
+    });
+    /* backward, as chol_solve */
+    auto load_row = [&](auto lc, float(&lv)[R]) {
+        constexpr int l = lc;
+        sfor<0, l / TS + 1>([&](auto r) {
+            const int i = t + TS * r;
+            lv[r] = L[(i < l) ? l * (l + 1) / 2 + i : ZERO];
+        });
+    };
+    float lv[R];
+    load_row(std::integral_constant<int, N - 1>{}, lv);
+    sfor_down<0, N>([&](auto lc) {
+        constexpr int l = lc;
+        constexpr int rl = l / TS, tl = l % TS;
+        float lvn[R];
+        if constexpr (l > 0) load_row(std::integral_constant<int, l - 1>{}, lvn);
+        dpp_ready_from<0>(b[rl]);
+        sfor<0, NRB>([&](auto q) {
+            sfor<0, rl>([&](auto r) { fnma_bcr<tl>(b[r][q], b[rl][q], lv[r]); });
+            fnma_bcr<tl>(b[rl][q], b[rl][q], lv[rl]);
+        });
+        if constexpr (l > 0) {
+            sfor<0, R>([&](auto r) { lv[r] = lvn[r]; });
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    sfor<0, R>([&](auto r) {
+        sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * dinv[r]; });
+    });
 }
 
 /* max over the 16 lanes of the team, in every lane (mirror butterfly) */

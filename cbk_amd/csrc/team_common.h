@@ -84,6 +84,66 @@ template <int S> __device__ __forceinline__ void fma_bc(float &acc, float x, flo
         : "v"(x), "v"(a), "i"(S));
 }
 
+/* fnma_bc / fma_bc without their s_nop, for broadcast sources made ready by
+ * dpp_ready: a sweep or an update whose source registers do not change
+ * between the reads pays the two wait states once instead of per fma. */
+template <int S> __device__ __forceinline__ void fnma_bcr(double &acc, double x, double a)
+{
+    asm("v_fmac_f64_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(x), "v"(a), "i"(S));
+}
+template <int S> __device__ __forceinline__ void fnma_bcr(float &acc, float x, float a)
+{
+    asm("v_fmac_f32_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(x), "v"(a), "i"(S));
+}
+
+/* Two wait states after the last VALU write of the registers named, before
+ * any DPP read of them that follows: the s_nop sits in an asm that names them
+ * as outputs, so no write of them can be scheduled after it. (Each call
+ * covers only its own operands.) tests/test_dpp_hazards.py checks the built
+ * code for every DPP read. */
+template <class T> __device__ __forceinline__ void dpp_ready(T &a)
+{
+    asm volatile("s_nop 1" : "+v"(a));
+}
+template <class T> __device__ __forceinline__ void dpp_ready(T &a, T &b)
+{
+    asm volatile("s_nop 1" : "+v"(a), "+v"(b));
+}
+template <class T> __device__ __forceinline__ void dpp_ready(T &a, T &b, T &c, T &d)
+{
+    asm volatile("s_nop 1" : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
+}
+/* entries B..R-1 of x */
+template <int B, class T, int R> __device__ __forceinline__ void dpp_ready_from(T (&x)[R])
+{
+    constexpr int n = R - B;
+    if constexpr (n == 1) dpp_ready(x[B]);
+    else if constexpr (n == 2) dpp_ready(x[B], x[B + 1]);
+    else if constexpr (n == 3) {
+        dpp_ready(x[B], x[B + 1]);
+        dpp_ready(x[B + 2]);
+    }
+    else if constexpr (n >= 4) {
+        dpp_ready(x[B], x[B + 1], x[B + 2], x[B + 3]);
+        if constexpr (n > 4) dpp_ready_from<B + 4>(x);
+    }
+}
+
+/* column J of a, rows B..R-1 */
+template <int B, int J, class T, int R, int C> __device__ __forceinline__ void dpp_ready_col(T (&a)[R][C])
+{
+    constexpr int n = R - B;
+    if constexpr (n == 1) dpp_ready(a[B][J]);
+    else if constexpr (n >= 2) {
+        dpp_ready(a[B][J], a[B + 1][J]);
+        if constexpr (n > 2) dpp_ready_col<B + 2, J>(a);
+    }
+}
+
 /* the value of x in lane l of the wavefront (l wave-uniform) */
 __device__ __forceinline__ double rdlane(double x, int l)
 {

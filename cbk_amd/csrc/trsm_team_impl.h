@@ -93,12 +93,13 @@ trsm_team_kernel(TrsmArgs<T> p)
                     b[rk][q] = (t == tk) ? b[rk][q] / dg[rk] : b[rk][q];
                 });
             }
+            dpp_ready_from<0>(b[rk]);
             sfor<0, NRB>([&](auto q) {
                 /* rows i <= k of x_k's own row block keep their value: an fma
                  * with a zero coefficient would still turn a -0 into +0 */
                 T own = b[rk][q];
-                fnma_bc<tk>(own, b[rk][q], a[rk][k]);
-                sfor<rk + 1, R>([&](auto r) { fnma_bc<tk>(b[r][q], b[rk][q], a[r][k]); });
+                fnma_bcr<tk>(own, b[rk][q], a[rk][k]);
+                sfor<rk + 1, R>([&](auto r) { fnma_bcr<tk>(b[r][q], b[rk][q], a[r][k]); });
                 b[rk][q] = (t > tk) ? own : b[rk][q];
             });
             __builtin_amdgcn_sched_barrier(0);
