@@ -58,10 +58,43 @@ constexpr int kTeamsPerBlock = 4;
 
 template <int N> hipError_t launch_n(const MixedArgs &args);
 
+/* whether c holds in any lane of the calling lane's team (a wave-wide ballot,
+ * then the team's 16 bits) */
+__device__ __forceinline__ bool team_any(bool c)
+{
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(c);
+    return ((m >> (threadIdx.x & 48u)) & 0xFFFFull) != 0;
+}
+
+/* The largest of each of 4 per-lane values over the team, in every lane,
+ * NaN-propagating: a transposed butterfly -- lane pairs (t, t^1) exchange
+ * the half of the values each does not keep, pairs (t, t^2) the quarter, then
+ * rotations by 4 and 8 combine the four lanes that keep the same value --
+ * and one broadcast per value back. 5 exchanges instead of 16. */
+__device__ __forceinline__ void team_max4(double (&x)[4], int t)
+{
+    const bool b0 = (t & 1) != 0, b1 = (t & 2) != 0;
+    double k0 = b0 ? x[2] : x[0], k1 = b0 ? x[3] : x[1];
+    const double s0 = b0 ? x[0] : x[2], s1 = b0 ? x[1] : x[3];
+    k0 = fmax_nan(k0, dpp<0xB1>(s0)); /* quad_perm [1,0,3,2] */
+    k1 = fmax_nan(k1, dpp<0xB1>(s1));
+    double k = b1 ? k1 : k0;
+    const double s = b1 ? k0 : k1;
+    k = fmax_nan(k, dpp<0x4E>(s));    /* quad_perm [2,3,0,1] */
+    k = fmax_nan(k, dpp<0x124>(k));   /* row_ror:4 */
+    k = fmax_nan(k, dpp<0x128>(k));   /* row_ror:8 */
+    /* lane b0 + 2 b1 now holds value 2 b0 + b1 */
+    x[0] = bcast<0>(k);
+    x[1] = bcast<2>(k);
+    x[2] = bcast<1>(k);
+    x[3] = bcast<3>(k);
+}
+
 template <int N, int NRB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N <= 16 ? 4 : 2, N <= 16 ? 4 : 2))) void
 mixed_posv_kernel(MixedArgs p)
 {
+    static_assert(NRB == 4, "the convergence test reduces four columns at once");
     constexpr int R = rows_per_lane<N>();
     /* one team region, used as FP32 L during refinement, FP64 L on fallback */
     __shared__ double lds[kTeamsPerBlock * chol_lds<N>()];
@@ -124,7 +157,7 @@ mixed_posv_kernel(MixedArgs p)
     anrm = team_max(anrm);
     const double cte = anrm * 0x1p-53 * __builtin_sqrt((double)N);
     /* any lane of the team overflowing sends the whole matrix to FP64 */
-    const bool a_over = team_max(over ? 1.0 : 0.0) > 0.0;
+    const bool a_over = team_any(over);
 
     int iter = 0;
     bool done = false;
@@ -159,7 +192,7 @@ mixed_posv_kernel(MixedArgs p)
     sfor<0, R>([&](auto r) {
         sfor<0, NRB>([&](auto q) { b_over |= __builtin_fabs(b64[r][q]) > 3.4028234663852886e38; });
     });
-    if (team_max(b_over ? 1.0 : 0.0) > 0.0) {
+    if (team_any(b_over)) {
         iter = -2;
         ready = false;
     }
@@ -181,20 +214,24 @@ mixed_posv_kernel(MixedArgs p)
                         sfor<0, R>([&](auto r) { fnma_bcr<c % TS>(res[r][q], x64[c / TS][q], a64[r][c]); });
                     });
                 });
-                /* every column: ||r||_inf <= ||x||_inf * cte (NaN fails) */
-                bool conv = true;
+                /* every column: ||r||_inf <= ||x||_inf * cte, tested per row
+                 * against the team's ||x||_inf (a NaN fails either way) */
+                double xm[NRB];
                 sfor<0, NRB>([&](auto q) {
-                    double rn = 0.0, xn = 0.0;
+                    xm[q] = 0.0;
                     sfor<0, R>([&](auto r) {
-                        if (t + TS * r < N) {
-                            rn = fmax_nan(rn, __builtin_fabs(res[r][q]));
-                            xn = fmax_nan(xn, __builtin_fabs(x64[r][q]));
-                        }
+                        if (t + TS * r < N) xm[q] = fmax_nan(xm[q], __builtin_fabs(x64[r][q]));
                     });
-                    rn = team_max(rn);
-                    xn = team_max(xn);
-                    if (q < p.nrhs && !(rn <= xn * cte)) conv = false;
                 });
+                team_max4(xm, t);
+                bool fail = false;
+                sfor<0, NRB>([&](auto q) {
+                    sfor<0, R>([&](auto r) {
+                        if (q < p.nrhs && t + TS * r < N)
+                            fail = fail || !(__builtin_fabs(res[r][q]) <= xm[q] * cte);
+                    });
+                });
+                const bool conv = !team_any(fail);
                 if (conv) {
                     iter = it;
                     done = true;
@@ -211,7 +248,7 @@ mixed_posv_kernel(MixedArgs p)
                         w[r][q] = (float)res[r][q];
                     });
                 });
-                if (team_max(rover ? 1.0 : 0.0) > 0.0) {
+                if (team_any(rover)) {
                     iter = -2;
                     break;
                 }
