@@ -78,13 +78,24 @@ trsm_team_kernel(TrsmArgs<T> p)
         dg[r] = bload(T(), ra, doff, 0);
     });
 
+    /* right-hand sides in blocks of NRB; the next block's loads are issued
+     * before the current block's sweep, so their latency hides behind it */
+    auto load_block = [&](int q0, T(&bb)[R][NRB]) {
+        sfor<0, NRB>([&](auto q) {
+            const unsigned col = (q0 + q < p.nrhs) ? (q0 + q) * b_col : kOOB;
+            sfor<0, R>([&](auto r) { bb[r][q] = bload(T(), rb, b_row[r], col); });
+        });
+    };
+    T bn[R][NRB];
+    if (p.nrhs > 0) load_block(0, bn);
     for (int q0 = 0; q0 < p.nrhs; q0 += NRB) {
         unsigned bcol[NRB];
         sfor<0, NRB>([&](auto q) { bcol[q] = (q0 + q < p.nrhs) ? (q0 + q) * b_col : kOOB; });
         T b[R][NRB];
         sfor<0, R>([&](auto r) {
-            sfor<0, NRB>([&](auto q) { b[r][q] = bload(T(), rb, b_row[r], bcol[q]) * p.alpha; });
+            sfor<0, NRB>([&](auto q) { b[r][q] = bn[r][q] * p.alpha; });
         });
+        if (q0 + NRB < p.nrhs) load_block(q0 + NRB, bn);
         sfor<0, N>([&](auto kc) {
             constexpr int k = kc;
             constexpr int rk = k / TS, tk = k % TS;
