@@ -99,6 +99,13 @@ hipError_t launch_mixed_posv_team(bool up, int n, int nrhs, Operand<double> A,
                                   Operand<double> B, Operand<double> X, int V, int nm,
                                   int itermax, int *iter, int *info);
 
+/* QR / overdetermined least squares with the columns of [F | B] in the 16
+ * lanes of a team (qr_cols.hip): p <= 32, q + nrhs <= 16, V >= 4 */
+bool qr_cols_supports(int p, int q, int nrhs);
+template <class T>
+hipError_t launch_qr_cols(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
+                          Operand<T> B, int V, int nm, int *info);
+
 /* shapes served by the QR team kernel: the tall view p x q (p >= q) */
 bool qr_team_supports(int p, int q, int nrhs, bool fp64);
 

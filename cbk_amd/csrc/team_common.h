@@ -100,6 +100,19 @@ template <int S> __device__ __forceinline__ void fnma_bcr(float &acc, float x, f
         : "v"(x), "v"(a), "i"(S));
 }
 
+template <int S> __device__ __forceinline__ void fma_bcr(double &acc, double x, double a)
+{
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(x), "v"(a), "i"(S));
+}
+template <int S> __device__ __forceinline__ void fma_bcr(float &acc, float x, float a)
+{
+    asm("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(x), "v"(a), "i"(S));
+}
+
 /* Two wait states after the last VALU write of the registers named, before
  * any DPP read of them that follows: the s_nop sits in an asm that names them
  * as outputs, so no write of them can be scheduled after it. (Each call
