@@ -120,8 +120,8 @@ def backward_error(torch, A, X, B):
 def load_traffic():
     """HBM bytes per launch of the two kernels, measured with rocprofv3 --pmc
     FETCH_SIZE / WRITE_SIZE on this benchmark and scaled by the same counters on
-    a plain copy of known size (profiles/dposv_k4_traffic.json), or None"""
-    path = os.path.join(ROOT, "profiles", "dposv_k4_traffic.json")
+    a plain copy of known size (profiles/kernel_traffic.json), or None"""
+    path = os.path.join(ROOT, "profiles", "kernel_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -605,8 +605,8 @@ def main():
     }
     traffic = load_traffic()
     if traffic:
-        result["roofline"]["traffic"] = traffic["mixed_posv_kernel"]["bytes_per_launch"]
-        result["plain_fp64"]["traffic_bytes_per_launch"] = traffic["sym_team_kernel"]["bytes_per_launch"]
+        result["roofline"]["traffic"] = traffic["mixed_posv_kernel_dsposv_n32_nrhs4"]["bytes_per_launch"]
+        result["plain_fp64"]["traffic_bytes_per_launch"] = traffic["sym_team_kernel_dposv_n32_nrhs4"]["bytes_per_launch"]
     if args.calibrate:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)

@@ -103,11 +103,13 @@ __global__ __launch_bounds__(256) void gels_team_kernel(BatchArgs<T> p)
                     sfor<kk + 1, MMAX>([&](auto i) { f[i] = f[i] * h.inv; });
                     f[kk] = h.rdiag;
                 }
+                /* the broadcast sources f[kk+1..] of lane kk are final here */
+                dpp_ready_from<kk + 1>(f);
                 T wv = f[kk];
-                sfor<kk + 1, MMAX>([&](auto i) { fma_bc<kk>(wv, f[i], f[i]); });
+                sfor<kk + 1, MMAX>([&](auto i) { fma_bcr<kk>(wv, f[i], f[i]); });
                 wv = (c > kk) ? wv * h.tau : T(0);
                 f[kk] = f[kk] - wv;
-                sfor<kk + 1, MMAX>([&](auto i) { fnma_bc<kk>(f[i], f[i], wv); });
+                sfor<kk + 1, MMAX>([&](auto i) { fnma_bcr<kk>(f[i], f[i], wv); });
             }
             __builtin_amdgcn_sched_barrier(0);
         });

@@ -17,7 +17,7 @@
  *
  * The FP32 factor (chol_factor_fast: hardware reciprocal square root) and the
  * FP32 solves are the team building blocks of team_chol.h; the solves read the
- * factor from the team's LDS (chol_solve_split), so that during refinement the
+ * factor from the team's LDS (chol_solve_cols), so that during refinement the
  * registers hold the FP64 A, B, X and the residual only, and B is loaded after
  * the factor: two wavefronts fit a SIMD at n = 32, four at n = 16. The
  * symmetric residual sums A(i,c) x_c over c ascending with one FP64 fma each.
@@ -96,8 +96,10 @@ mixed_posv_kernel(MixedArgs p)
 {
     static_assert(NRB == 4, "the convergence test reduces four columns at once");
     constexpr int R = rows_per_lane<N>();
-    /* one team region, used as FP32 L during refinement, FP64 L on fallback */
-    __shared__ double lds[kTeamsPerBlock * chol_lds<N>()];
+    /* a team's region: the FP32 column copy during refinement, the FP64
+     * packed rows of chol_prepare on the fallback */
+    constexpr int TW = chol_lds<N>() > (cols_lds<N>() + 1) / 2 ? chol_lds<N>() : (cols_lds<N>() + 1) / 2;
+    __shared__ double lds[kTeamsPerBlock * TW];
 
     const int t = threadIdx.x & (TS - 1);
     const int team = threadIdx.x / TS;                    /* 0..15 in the block */
@@ -162,7 +164,7 @@ mixed_posv_kernel(MixedArgs p)
     int iter = 0;
     bool done = false;
     bool ready = false; /* the FP32 factor is in LDS */
-    float *L32 = reinterpret_cast<float *>(lds + team * chol_lds<N>());
+    float *L32 = reinterpret_cast<float *>(lds + team * TW);
     float dinv32[R];
     if (!a_over) {
         float a32[R][N], dg32[R];
@@ -174,7 +176,7 @@ mixed_posv_kernel(MixedArgs p)
             iter = -3;
         }
         else {
-            chol_prepare_split<N>(a32, dg32, dinv32, L32, t);
+            chol_prepare_cols<N>(a32, dg32, dinv32, L32, t);
             ready = true;
         }
     }
@@ -202,7 +204,7 @@ mixed_posv_kernel(MixedArgs p)
         {
             float w[R][NRB];
             sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { w[r][q] = (float)b64[r][q]; }); });
-            chol_solve_split<N, NRB>(dinv32, w, L32, t);
+            chol_solve_cols<N, NRB>(dinv32, w, L32, t);
             sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { x64[r][q] = (double)w[r][q]; }); });
             for (int it = 0;; ++it) {
                 /* r = b - A x, FP64 */
@@ -252,7 +254,7 @@ mixed_posv_kernel(MixedArgs p)
                     iter = -2;
                     break;
                 }
-                chol_solve_split<N, NRB>(dinv32, w, L32, t);
+                chol_solve_cols<N, NRB>(dinv32, w, L32, t);
                 sfor<0, R>([&](auto r) {
                     sfor<0, NRB>([&](auto q) { x64[r][q] = x64[r][q] + (double)w[r][q]; });
                 });
@@ -278,9 +280,9 @@ mixed_posv_kernel(MixedArgs p)
             bstore(dg[r], ra, doff, 0);
         });
         if (dinfo == 0) {
-            chol_prepare<double, N, false>(a, dg, dinv, lds + team * chol_lds<N>(), t);
+            chol_prepare<double, N, false>(a, dg, dinv, lds + team * TW, t);
             sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { x64[r][q] = b64[r][q]; }); });
-            chol_solve<double, N, NRB, false>(a, dinv, x64, lds + team * chol_lds<N>(), t);
+            chol_solve<double, N, NRB, false>(a, dinv, x64, lds + team * TW, t);
         }
     }
     if (mat < p.nm && t == 0) {

@@ -285,57 +285,57 @@ __device__ __forceinline__ void chol_solve(const T (&a)[rows_per_lane<N>()][N],
 
 /* The FP32 Cholesky solve with both sweeps reading L from LDS, so the
  * factor's registers are free once it is prepared (the mixed kernel keeps
- * the FP64 A in registers for its residuals). The team's region holds two
- * FP32 copies: at L the packed rows of chol_prepare (L(i,c)/L(i,i) at
- * i(i+1)/2 + c, for the backward sweep), at L + chol_lds<N>() the packed
- * columns scaled by 1/L(c,c) (L(i,c) at cN - c(c-1)/2 + i - c, for the
- * forward sweep: lanes read consecutive words). Both end in a 0.0 slot. */
-template <int N> constexpr int fwd_at(int i, int c) { return c * N - c * (c - 1) / 2 + (i - c); }
+ * the FP64 A in registers for its residuals). One copy serves both sweeps:
+ * the columns of L scaled by 1/L(c,c), each column c stored for the row
+ * blocks from its own down (rows 16*(c/16) .. 16*R-1, zeros on and above the
+ * diagonal and in rows past n), so every read is a lane's base plus a
+ * compile-time offset and needs no mask:
+ *   forward  L y = b as (unit lower L D^-1) v = b, v = D y: step i reads
+ *            column i at the lane's own rows;
+ *   backward L^T x = y as x = D^-1 y - (L D^-1)^T x, from v: step l reads
+ *            row l across the lane's columns i = t + 16r (column i's slot l).
+ * D = diag(L). */
+template <int N> constexpr int cols_at(int c)
+{
+    int s = 0;
+    for (int k = 0; k < c; ++k) s += TS * (rows_per_lane<N>() - k / TS);
+    return s;
+}
+/* floats of the copy */
+template <int N> constexpr int cols_lds() { return cols_at<N>(N); }
 
 template <int N>
-__device__ __forceinline__ void chol_prepare_split(const float (&a)[rows_per_lane<N>()][N],
-                                                   const float (&dg)[rows_per_lane<N>()],
-                                                   float (&dinv)[rows_per_lane<N>()], float *L,
-                                                   int t)
+__device__ __forceinline__ void chol_prepare_cols(const float (&a)[rows_per_lane<N>()][N],
+                                                  const float (&dg)[rows_per_lane<N>()],
+                                                  float (&dinv)[rows_per_lane<N>()], float *L,
+                                                  int t)
 {
     constexpr int R = rows_per_lane<N>();
-    constexpr int ZERO = N * (N + 1) / 2, SINK = ZERO + 1;
-    float *Lc = L + chol_lds<N>();
     sfor<0, R>([&](auto r) { dinv[r] = 1.0f / dg[r]; });
     sfor<0, N>([&](auto c) {
         const float dc = bcast<c % TS>(dinv[c / TS]);
         sfor<c / TS, R>([&](auto r) {
-            const int i = t + TS * r;
-            const bool below = (i < N) && (c < TS * r || c < i);
-            L[below ? i * (i + 1) / 2 + c : SINK] = a[r][c] * dinv[r];
-            Lc[below ? fwd_at<N>(i, c) : SINK] = a[r][c] * dc;
+            const int row = t + TS * r;
+            L[cols_at<N>(c) + TS * (r - c / TS) + t] = (c < row && row < N) ? a[r][c] * dc : 0.0f;
         });
     });
-    if (t == 0) {
-        L[ZERO] = 0.0f;
-        Lc[ZERO] = 0.0f;
-    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int N, int NRB>
-__device__ __forceinline__ void chol_solve_split(const float (&dinv)[rows_per_lane<N>()],
-                                                 float (&b)[rows_per_lane<N>()][NRB],
-                                                 const float *L, int t)
+__device__ __forceinline__ void chol_solve_cols(const float (&dinv)[rows_per_lane<N>()],
+                                                float (&b)[rows_per_lane<N>()][NRB],
+                                                const float *L, int t)
 {
     constexpr int R = rows_per_lane<N>();
-    constexpr int ZERO = N * (N + 1) / 2;
     constexpr int D = 4; /* forward coefficients are read D steps ahead */
-    const float *Lc = L + chol_lds<N>();
-    /* forward: step i needs L(t+16r, i)/L(i,i) for the rows below i */
+    const float *Lt = L + t;
+    /* forward: step i reads column i at rows t + 16r, r >= i/16 */
     auto load_col = [&](auto ic, float(&cv)[R]) {
         constexpr int i = ic;
-        sfor<i / TS, R>([&](auto r) {
-            const int row = t + TS * r;
-            cv[r] = Lc[(row > i && row < N) ? fwd_at<N>(row, i) : ZERO];
-        });
+        sfor<i / TS, R>([&](auto r) { cv[r] = Lt[cols_at<N>(i) + TS * (r - i / TS)]; });
     };
     float ring[D][R];
     sfor<0, cmin(D, N)>([&](auto ic) { load_col(ic, ring[ic]); });
@@ -352,16 +352,21 @@ __device__ __forceinline__ void chol_solve_split(const float (&dinv)[rows_per_la
         });
         __builtin_amdgcn_sched_barrier(0);
     });
+    /* x starts from D^-2 v */
     sfor<0, R>([&](auto r) {
-        sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * dinv[r]; });
+        const float s = dinv[r] * dinv[r];
+        sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * s; });
     });
-    /* backward, as chol_solve */
+    /* backward: column i = t + 16r starts at cols_at(i) with row 16r */
+    int base[R];
+    sfor<0, R>([&](auto r) {
+        int full = 0;
+        sfor<0, r>([&](auto k) { full += TS * TS * (R - k); });
+        base[r] = full + TS * (R - r) * t - TS * r;
+    });
     auto load_row = [&](auto lc, float(&lv)[R]) {
         constexpr int l = lc;
-        sfor<0, l / TS + 1>([&](auto r) {
-            const int i = t + TS * r;
-            lv[r] = L[(i < l) ? l * (l + 1) / 2 + i : ZERO];
-        });
+        sfor<0, l / TS + 1>([&](auto r) { lv[r] = L[base[r] + l]; });
     };
     float lv[R];
     load_row(std::integral_constant<int, N - 1>{}, lv);
@@ -379,9 +384,6 @@ __device__ __forceinline__ void chol_solve_split(const float (&dinv)[rows_per_la
             sfor<0, R>([&](auto r) { lv[r] = lvn[r]; });
         }
         __builtin_amdgcn_sched_barrier(0);
-    });
-    sfor<0, R>([&](auto r) {
-        sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * dinv[r]; });
     });
 }
 
