@@ -141,14 +141,29 @@ __global__ __launch_bounds__(256) void gels_team_kernel(BatchArgs<T> p)
     }
 }
 
+/* LDS writes of some lanes visible to every lane of the same wavefront (its
+ * LDS operations complete in order; the fences keep the compiler from moving
+ * accesses across) */
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /* ---- wavefront kernel: n + nrhs <= 64, m <= MMAX ----------------------------
- * The reflector loop runs at run time, so each lane keeps a window of its
- * column that starts at the current reflector's row: f[i] = row kk + i. Row kk
- * is final once reflector kk is applied; it is stored and the window shifts
- * up by one (zeros enter at the bottom, where rows >= m already are zero). The
- * back substitution walks a reversed window up from row n-1 the same way.
- * Every index into f is a compile-time constant, and rows past the end are
- * zeros that every update leaves zero. */
+ * The reflector loop runs at run time in steps of G = 8: each lane keeps a
+ * window of its column that starts at row k0, f[i] = row k0 + i, and the 8
+ * reflectors k0 .. k0+7 pivot on window rows 0..7 (compile-time indices);
+ * then the window shifts up by 8 (zeros enter at the bottom, where rows >= m
+ * already are zero). The reflector's body v_i, which every lane needs, is
+ * written once to the wavefront's LDS row by lane kk and read back by all
+ * lanes at the same address (a broadcast read), in place of two v_readlane
+ * per row for the dot product and two more for the update. The back
+ * substitution walks a reversed window up from row n-1 the same way, R's
+ * column l broadcast through LDS from lane l. Every index into the windows is
+ * a compile-time constant, and rows past the end are zeros that every update
+ * leaves zero. */
 /* x, as a value the compiler must take to depend on dep. A v_readlane of x
  * then cannot be merged with an earlier one or hoisted above dep; chaining
  * dep to the result four rows back keeps about four broadcast values (eight
@@ -185,8 +200,13 @@ template <int B, int E, class F> __device__ __forceinline__ void live_rows(int l
 }
 
 template <class T, int MMAX>
-__global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (MMAX > 32 ? 2 : 4) : (MMAX > 32 ? 4 : 6)))) void
+gels_wave_kernel(BatchArgs<T> p)
 {
+    constexpr int G = 8; /* reflectors per window position */
+    __shared__ T vshare[4][MMAX];
+    T *vb = vshare[wave_in_block()]; /* this wavefront's broadcast row */
     const int c = threadIdx.x & 63;
     for (int round = 0;; ++round) {
         unsigned kq = 0;
@@ -215,7 +235,7 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
                        : col;
         const long long ors = xout ? p.x_rs : rs;
 
-        T f[MMAX];
+        T f[MMAX]; /* window: f[i] = row k0 + i of the lane's column */
         sfor<0, MMAX>([&](auto i) { f[i] = T(0); });
         if (has) {
             sfor<0, MMAX>([&](auto i) {
@@ -224,38 +244,50 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
         }
 
         T own = T(0); /* R(c,c) of an A lane */
-        for (int kk = 0; kk < n; ++kk) {
-            const int live = m - kk; /* window rows that can be nonzero */
-            T tl = T(0);
-            live_rows<1, MMAX>(live, [&](auto i) { tl = fmaT(f[i], f[i], tl); });
-            const Larfg<T> h = larfg(rdlane(f[0], kk), rdlane(tl, kk));
-            if (tau && c == 0) tau[(long long)kk * p.cV] = h.tau;
-            if (c == kk) {
-                live_rows<1, MMAX>(live, [&](auto i) { f[i] = f[i] * h.inv; });
-                f[0] = h.rdiag;
+        for (int k0 = 0; k0 < n; k0 += G) {
+            const int live = m - k0; /* window rows that can be nonzero */
+            sfor<0, G>([&](auto jc) {
+                constexpr int j = jc; /* reflector kk = k0 + j pivots on window row j */
+                const int kk = k0 + j;
+                if (kk < n) {
+                    T t4[4] = {T(0), T(0), T(0), T(0)};
+                    live_rows<j + 1, MMAX>(live, [&](auto i) { t4[i % 4] = fmaT(f[i], f[i], t4[i % 4]); });
+                    const T tl = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+                    const Larfg<T> h = larfg(rdlane(f[j], kk), rdlane(tl, kk));
+                    if (tau && c == 0) tau[(long long)kk * p.cV] = h.tau;
+                    if (c == kk) {
+                        live_rows<j + 1, MMAX>(live, [&](auto i) {
+                            f[i] = f[i] * h.inv;
+                            vb[i] = f[i];
+                        });
+                        f[j] = h.rdiag;
+                    }
+                    wave_lds_sync();
+                    /* w = f[j] + sum v_i f[i], four partial sums; v_i from the
+                     * wavefront's LDS row, read by every lane at once */
+                    T acc[4] = {f[j], T(0), T(0), T(0)};
+                    live_rows<j + 1, MMAX>(live, [&](auto i) { acc[i % 4] = fmaT(vb[i], f[i], acc[i % 4]); });
+                    T wv = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+                    wv = (c > kk) ? wv * h.tau : T(0);
+                    f[j] = f[j] - wv;
+                    live_rows<j + 1, MMAX>(live, [&](auto i) { f[i] = fmaT(-vb[i], wv, f[i]); });
+                    own = (c == kk) ? f[j] : own;
+                    if (has) ocol[(long long)kk * ors] = f[j];
+                    wave_lds_sync(); /* every read of vb before the next write */
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            if (k0 + G < n) {
+                live_rows<0, MMAX - G>(live, [&](auto i) { f[i] = f[i + G]; });
+                sfor<MMAX - G, MMAX>([&](auto i) { f[i] = T(0); });
             }
-            /* w = f[0] + sum v_i f[i], four partial sums */
-            T acc[4] = {f[0], T(0), T(0), T(0)};
-            live_rows<1, MMAX>(live, [&](auto i) {
-                const T fi = (i > 4) ? after(f[i], acc[i % 4]) : f[i];
-                acc[i % 4] = fmaT(rdlane(fi, kk), fi, acc[i % 4]);
-            });
-            T wv = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-            wv = (c > kk) ? wv * h.tau : T(0);
-            f[0] = f[0] - wv;
-            live_rows<1, MMAX>(live, [&](auto i) {
-                const T fi = after(f[i], (i > 4) ? f[i - 4] : wv);
-                f[i] = fmaT(-rdlane(fi, kk), wv, fi);
-            });
-            own = (c == kk) ? f[0] : own;
-            if (has) ocol[kk * ors] = f[0];
-            live_rows<0, MMAX - 1>(live, [&](auto i) { f[i] = f[i + 1]; });
-            f[MMAX - 1] = T(0);
         }
         if (has && !xout) {
             /* rows n..m-1: Q^T B below the solution, reflector bodies of A */
+            const int kb = n > 0 ? ((n - 1) / G) * G : 0; /* the last window's first row */
             sfor<0, MMAX>([&](auto i) {
-                if (i < m - n) col[(n + i) * rs] = f[i];
+                const int row = kb + i;
+                if (row >= n && row < m) col[(long long)row * rs] = f[i];
             });
         }
 
@@ -270,24 +302,38 @@ __global__ __launch_bounds__(256) void gels_wave_kernel(BatchArgs<T> p)
         if (c == 0 && p.info && d.idx < p.ninfo) p.info[d.idx] = bad;
         if (bad != 0 && p.info) continue; /* LAPACK dgels stops; without info, solve as gels_compact */
 
-        /* R X = (Q^T B)(0:n): g[j] = row l - j of the lane's column at step l */
+        /* R X = (Q^T B)(0:n), column-oriented from row n-1 up, G steps per
+         * window: g[i] = row lb - i of the lane's column (R's column for an A
+         * lane, the right-hand side for a B lane); step l = lb - j takes
+         * x_l = b_l / R(l,l), then b_r -= R(r,l) x_l for rows r < l, with
+         * R(r,l) broadcast from lane l's window through LDS */
         T g[MMAX];
-        sfor<0, MMAX>([&](auto j) { g[j] = T(0); });
+        sfor<0, MMAX>([&](auto i) { g[i] = T(0); });
         if (has) {
-            sfor<0, MMAX>([&](auto j) {
-                if (j < n) g[j] = ocol[(n - 1 - j) * ors];
+            sfor<0, MMAX>([&](auto i) {
+                if (i < n) g[i] = ocol[(long long)(n - 1 - i) * ors];
             });
         }
-        for (int l = n - 1; l >= 0; --l) {
-            const T dl = rdlane(own, l);
-            const T x = isB ? g[0] / dl : T(0);
-            if (isB) ocol[l * ors] = x;
-            live_rows<1, MMAX>(l + 1, [&](auto j) {
-                const T gj = after(g[j], (j > 4) ? g[j - 4] : x);
-                g[j] = fmaT(-rdlane(gj, l), x, gj);
+        for (int lb = n - 1; lb >= 0; lb -= G) {
+            sfor<0, G>([&](auto jc) {
+                constexpr int j = jc;
+                const int l = lb - j;
+                if (l >= 0) {
+                    const T dl = rdlane(own, l);
+                    const T x = isB ? g[j] / dl : T(0);
+                    if (isB) ocol[(long long)l * ors] = x;
+                    if (c == l) live_rows<j + 1, MMAX>(lb + 1, [&](auto i) { vb[i] = g[i]; });
+                    wave_lds_sync();
+                    live_rows<j + 1, MMAX>(lb + 1, [&](auto i) { g[i] = fmaT(-vb[i], x, g[i]); });
+                    g[j] = isB ? x : g[j];
+                    wave_lds_sync();
+                }
+                __builtin_amdgcn_sched_barrier(0);
             });
-            live_rows<0, MMAX - 1>(l + 1, [&](auto j) { g[j] = g[j + 1]; });
-            g[MMAX - 1] = T(0);
+            if (lb - G >= 0) {
+                live_rows<0, MMAX - G>(lb + 1, [&](auto i) { g[i] = g[i + G]; });
+                sfor<MMAX - G, MMAX>([&](auto i) { g[i] = T(0); });
+            }
         }
     }
 }
