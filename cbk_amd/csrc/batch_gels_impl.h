@@ -215,9 +215,11 @@ gels_wave_kernel(BatchArgs<T> p)
             kq = __builtin_amdgcn_readfirstlane(kq);
         }
         else {
-            /* compact mode: one problem per wavefront */
+            /* compact mode: one problem per wavefront; blocks in XCD order, so
+             * the 16 consecutive matrices that share the lines of an element
+             * (V = 64) are read through one L2 */
             if (round > 0) return;
-            kq = blockIdx.x * 4u + (unsigned)wave_in_block();
+            kq = xcd_block(blockIdx.x) * 4u + (unsigned)wave_in_block();
         }
         if (kq >= (unsigned)p.count) return;
         if (p.mask && (kq >= (unsigned)p.ninfo || p.mask[kq] >= 0)) continue;
@@ -526,7 +528,7 @@ hipError_t launch_qr_wave(int p, int q, int nrhs, bool transposed, Operand<T> A,
     args.x = nullptr;
     args.x_rs = args.x_cs = 0;
     args.x_gs = 0;
-    const unsigned blocks = (unsigned)((args.count + 3) / 4);
+    const unsigned blocks = xcd_grid((unsigned)((args.count + 3) / 4));
     if (p <= 32)
         hipLaunchKernelGGL((batch::gels_wave_kernel<T, 32>), dim3(blocks), dim3(256), 0,
                            current_stream(), args);
