@@ -340,6 +340,150 @@ gels_wave_kernel(BatchArgs<T> p)
     }
 }
 
+/* ---- wavefront kernel, two problems per wavefront: n + nrhs <= 32 ----------
+ * Lanes 0-31 take one problem of the queue and lanes 32-63 the next, each
+ * half with the windows and LDS broadcasts of gels_wave_kernel (an LDS row
+ * per half). The loops run to the larger of the two bounds; a half whose
+ * problem is shorter idles through the extra steps with zero updates (its
+ * LDS row keeps finite values from its own earlier steps, or the zeros it
+ * starts with, so a zero weight leaves its columns unchanged). Rows that
+ * leave a window below a half's last reflector are final and stored then. */
+template <class T, int MMAX>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (MMAX > 32 ? 2 : 4) : (MMAX > 32 ? 4 : 6)))) void
+gels_wave_pair_kernel(BatchArgs<T> p)
+{
+    constexpr int G = 8;
+    __shared__ T vshare[4][2][MMAX];
+    const int lane = threadIdx.x & 63;
+    const int hf = lane >> 5, c = lane & 31;
+    T *vb = vshare[wave_in_block()][hf];
+    sfor<0, (MMAX + 31) / 32>([&](auto r) {
+        if (c + 32 * r < MMAX) vb[c + 32 * r] = T(0);
+    });
+    wave_lds_sync();
+    auto both = [&](int x, auto op) {
+        return op(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 32));
+    };
+    auto imax = [](int a, int b) { return a > b ? a : b; };
+    for (;;) {
+        unsigned k0q = 0;
+        if (lane == 0) k0q = atomicAdd(p.counter, 2u);
+        k0q = __builtin_amdgcn_readfirstlane(k0q);
+        if (k0q >= (unsigned)p.count) return;
+        const unsigned kq = k0q + (unsigned)hf;
+        const bool real = kq < (unsigned)p.count;
+        const BatchProblem d = p.desc[real ? kq : k0q];
+        const int m = real ? d.m : 0, n = real ? d.n : 0, w = real ? d.n + d.nrhs : 0;
+        const int nmax = both(n, imax), mmax = both(m, imax);
+        const bool isA = c < n, isB = c >= n && c < w;
+        const bool has = isA || isB;
+        T *col = isA ? p.a + d.a_off + (long long)c * d.lda
+                     : p.b + d.b_off + (long long)(isB ? c - n : 0) * d.ldb;
+
+        T f[MMAX]; /* window: f[i] = row k0 + i */
+        sfor<0, MMAX>([&](auto i) { f[i] = T(0); });
+        if (has) {
+            sfor<0, MMAX>([&](auto i) {
+                if (i < m) f[i] = col[i];
+            });
+        }
+        T own = T(0);
+        for (int k0 = 0; k0 < nmax; k0 += G) {
+            const int live = mmax - k0;
+            sfor<0, G>([&](auto jc) {
+                constexpr int j = jc;
+                const int kk = k0 + j;
+                if (kk < nmax) {
+                    const bool act = kk < n;
+                    T t4[4] = {T(0), T(0), T(0), T(0)};
+                    live_rows<j + 1, MMAX>(live, [&](auto i) { t4[i % 4] = fmaT(f[i], f[i], t4[i % 4]); });
+                    const T tl = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+                    const T x0 = hf ? rdlane(f[j], 32 + kk) : rdlane(f[j], kk);
+                    const T tt = hf ? rdlane(tl, 32 + kk) : rdlane(tl, kk);
+                    const Larfg<T> h = larfg(x0, tt);
+                    if (act && c == kk) {
+                        live_rows<j + 1, MMAX>(live, [&](auto i) {
+                            f[i] = f[i] * h.inv;
+                            vb[i] = f[i];
+                        });
+                        f[j] = h.rdiag;
+                    }
+                    wave_lds_sync();
+                    T acc[4] = {f[j], T(0), T(0), T(0)};
+                    live_rows<j + 1, MMAX>(live, [&](auto i) { acc[i % 4] = fmaT(vb[i], f[i], acc[i % 4]); });
+                    T wv = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+                    wv = (act && c > kk) ? wv * h.tau : T(0);
+                    f[j] = f[j] - wv;
+                    live_rows<j + 1, MMAX>(live, [&](auto i) { f[i] = fmaT(-vb[i], wv, f[i]); });
+                    own = (act && c == kk) ? f[j] : own;
+                    if (has && act) col[kk] = f[j];
+                    wave_lds_sync();
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            if (k0 + G < nmax) {
+                if (has) {
+                    sfor<0, G>([&](auto i) {
+                        const int row = k0 + i;
+                        if (row >= n && row < m) col[row] = f[i];
+                    });
+                }
+                live_rows<0, MMAX - G>(live, [&](auto i) { f[i] = f[i + G]; });
+                sfor<MMAX - G, MMAX>([&](auto i) { f[i] = T(0); });
+            }
+        }
+        if (has) {
+            const int kb = nmax > 0 ? ((nmax - 1) / G) * G : 0;
+            sfor<0, MMAX>([&](auto i) {
+                const int row = kb + i;
+                if (row >= n && row < m) col[row] = f[i];
+            });
+        }
+
+        int bad = 0;
+        for (int l = 0; l < nmax; ++l) {
+            const T o = hf ? rdlane(own, 32 + l) : rdlane(own, l);
+            if (bad == 0 && l < n && o == T(0)) bad = l + 1;
+        }
+        if (c == 0 && real && p.info) p.info[d.idx] = bad;
+        /* LAPACK dgels stops at a zero diagonal when info is wanted */
+        const int nb = (bad != 0 && p.info) ? 0 : n;
+        const int nbmax = both(nb, imax);
+
+        T g[MMAX]; /* window: g[i] = row lb - i */
+        sfor<0, MMAX>([&](auto i) { g[i] = T(0); });
+        if (has) {
+            sfor<0, MMAX>([&](auto i) {
+                const int row = nbmax - 1 - i;
+                if (row >= 0 && row < nb) g[i] = col[row];
+            });
+        }
+        for (int lb = nbmax - 1; lb >= 0; lb -= G) {
+            sfor<0, G>([&](auto jc) {
+                constexpr int j = jc;
+                const int l = lb - j;
+                if (l >= 0) {
+                    const bool act = l < nb;
+                    const T dl = hf ? rdlane(own, 32 + l) : rdlane(own, l);
+                    const T x = (isB && act) ? g[j] / dl : T(0);
+                    if (isB && act) col[l] = x;
+                    if (act && c == l) live_rows<j + 1, MMAX>(lb + 1, [&](auto i) { vb[i] = g[i]; });
+                    wave_lds_sync();
+                    live_rows<j + 1, MMAX>(lb + 1, [&](auto i) { g[i] = fmaT(-vb[i], x, g[i]); });
+                    g[j] = (isB && act) ? x : g[j];
+                    wave_lds_sync();
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            if (lb - G >= 0) {
+                live_rows<0, MMAX - G>(lb + 1, [&](auto i) { g[i] = g[i + G]; });
+                sfor<MMAX - G, MMAX>([&](auto i) { g[i] = T(0); });
+            }
+        }
+    }
+}
+
 /* ---- wavefront kernel, two arrays: n <= 64, nrhs <= 64, m <= 64 ------------
  * For n + nrhs > 64 the columns of [A | B] outnumber the lanes: lane c holds
  * A's column c (c < n) in f and B's column c (c < nrhs) in fb. Both take the
@@ -480,6 +624,8 @@ template <class T> hipError_t launch_class(int cls, const BatchArgs<T> &args, un
     case 3: hipLaunchKernelGGL((gels_wave_kernel<T, 32>), grid, block, 0, s, args); break;
     case 4: hipLaunchKernelGGL((gels_wave_kernel<T, 64>), grid, block, 0, s, args); break;
     case 5: hipLaunchKernelGGL((gels_wave2_kernel<T>), grid, block, 0, s, args); break;
+    case 7: hipLaunchKernelGGL((gels_wave_pair_kernel<T, 32>), grid, block, 0, s, args); break;
+    case 8: hipLaunchKernelGGL((gels_wave_pair_kernel<T, 64>), grid, block, 0, s, args); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -528,7 +674,7 @@ hipError_t launch_qr_wave(int p, int q, int nrhs, bool transposed, Operand<T> A,
     args.x = nullptr;
     args.x_rs = args.x_cs = 0;
     args.x_gs = 0;
-    const unsigned blocks = xcd_grid((unsigned)((args.count + 3) / 4));
+    const unsigned blocks = team::xcd_grid((unsigned)((args.count + 3) / 4));
     if (p <= 32)
         hipLaunchKernelGGL((batch::gels_wave_kernel<T, 32>), dim3(blocks), dim3(256), 0,
                            current_stream(), args);
@@ -548,6 +694,8 @@ template <class T> hipError_t gels_batch_occupancy(int cls, int *blocks)
     case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_kernel<T, 32>, 256, 0);
     case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_kernel<T, 64>, 256, 0);
     case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave2_kernel<T>, 256, 0);
+    case 7: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_pair_kernel<T, 32>, 256, 0);
+    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_pair_kernel<T, 64>, 256, 0);
     default: *blocks = 0; return hipErrorInvalidValue;
     }
 }

@@ -26,6 +26,7 @@ using cbk::BatchArgs;
 using cbk::BatchProblem;
 using cbk::kBatchClasses;
 using cbk::kBatchGeneral;
+using cbk::kBatchKernels;
 
 static_assert(CBK_BATCH_NCLASS == kBatchClasses, "class count");
 
@@ -80,7 +81,16 @@ double work_of(const cbk_gels_problem &p)
     return m * n * n - n * n * n / 3.0 + 2.0 * m * n * r + n * n * r;
 }
 
-int problems_per_block(int cls) { return cls <= 2 ? 16 : cls <= 5 ? 4 : 256; }
+/* the kernel a problem of class c runs on: the wavefront classes' problems
+ * with n + nrhs <= 32 go two per wavefront (kernels 7 and 8) */
+int kernel_of(int c, const cbk_gels_problem &p)
+{
+    return (c == 3 || c == 4) && p.n + p.nrhs <= 32 ? c + 4 : c;
+}
+/* the public class of kernel k */
+int class_of_kernel(int k) { return k >= kBatchClasses ? k - 4 : k; }
+
+int problems_per_block(int k) { return k <= 2 ? 16 : k <= 5 ? 4 : k == kBatchGeneral ? 256 : 8; }
 
 } /* namespace */
 
@@ -89,7 +99,7 @@ struct cbk_batch_plan {
     bool tran = false;
     int nprob = 0;
     std::vector<BatchProblem> table;
-    int begin[kBatchClasses + 1] = {};
+    int begin[kBatchKernels + 1] = {}; /* per kernel */
     long long tau_elems = 0;
     size_t table_off = kCounterBytes, tau_off = 0, bytes = 0;
     cbk_tuning tune{};
@@ -99,14 +109,14 @@ struct cbk_batch_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void *key[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipStream_t cap = nullptr, side[kBatchClasses] = {};
-    hipEvent_t fork = nullptr, join[kBatchClasses] = {};
+    hipStream_t cap = nullptr, side[kBatchKernels] = {};
+    hipEvent_t fork = nullptr, join[kBatchKernels] = {};
 
     ~cbk_batch_plan()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (int c = 0; c < kBatchClasses; ++c) {
+        for (int c = 0; c < kBatchKernels; ++c) {
             if (side[c]) (void)hipStreamDestroy(side[c]);
             if (join[c]) (void)hipEventDestroy(join[c]);
         }
@@ -122,7 +132,7 @@ hipError_t ensure_streams(cbk_batch_plan *pl)
     if (pl->cap) return hipSuccess;
     hipError_t e = hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming);
-    for (int c = 0; c < kBatchClasses && e == hipSuccess; ++c) {
+    for (int c = 0; c < kBatchKernels && e == hipSuccess; ++c) {
         e = hipStreamCreateWithFlags(&pl->side[c], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->join[c], hipEventDisableTiming);
     }
@@ -164,7 +174,7 @@ template <class T> hipError_t class_blocks(const cbk_batch_plan *pl, int c, unsi
         *blocks = (unsigned)need;
         return hipSuccess;
     }
-    int per_cu = pl->tune.blocks_per_cu[c];
+    int per_cu = pl->tune.blocks_per_cu[class_of_kernel(c)];
     hipError_t e = hipSuccess;
     if (per_cu <= 0) e = cbk::gels_batch_occupancy<T>(c, &per_cu);
     if (e != hipSuccess) return e;
@@ -192,8 +202,8 @@ hipError_t record_graph(cbk_batch_plan *pl, T *ap, T *bp, int *info, char *work)
 {
     hipError_t e = ensure_streams(pl);
     if (e != hipSuccess) return e;
-    unsigned blocks[kBatchClasses] = {};
-    for (int c = 0; c < kBatchClasses && e == hipSuccess; ++c)
+    unsigned blocks[kBatchKernels] = {};
+    for (int c = 0; c < kBatchKernels && e == hipSuccess; ++c)
         if (pl->begin[c + 1] > pl->begin[c]) e = class_blocks<T>(pl, c, &blocks[c]);
     if (e != hipSuccess) return e;
     if (pl->exec) {
@@ -206,9 +216,9 @@ hipError_t record_graph(cbk_batch_plan *pl, T *ap, T *bp, int *info, char *work)
     }
     e = hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    hipError_t le = cbk::launch_clear_u32(reinterpret_cast<unsigned *>(work), kBatchClasses, pl->cap);
+    hipError_t le = cbk::launch_clear_u32(reinterpret_cast<unsigned *>(work), kBatchKernels, pl->cap);
     if (le == hipSuccess) le = hipEventRecord(pl->fork, pl->cap);
-    for (int c = 0; c < kBatchClasses && le == hipSuccess; ++c) {
+    for (int c = 0; c < kBatchKernels && le == hipSuccess; ++c) {
         if (pl->begin[c + 1] == pl->begin[c]) continue;
         le = hipStreamWaitEvent(pl->side[c], pl->fork, 0);
         if (le == hipSuccess) le = launch_class<T>(pl, c, blocks[c], ap, bp, info, work, pl->side[c]);
@@ -273,7 +283,7 @@ hipError_t time_class(int cls, int m, int n, int nrhs, int count, int dev, float
     const long long sa = (long long)m * n, sb = (long long)m * nrhs;
     for (int k = 0; k < count; ++k)
         pl.table[k] = BatchProblem{m, n, nrhs, m, m, k, k * sa, k * sb, 0};
-    for (int c = 0; c <= kBatchClasses; ++c)
+    for (int c = 0; c <= kBatchKernels; ++c)
         pl.begin[c] = c <= cls ? 0 : count;
     pl.tau_off = pl.table_off + ((count * sizeof(BatchProblem) + 255) / 256) * 256;
     pl.bytes = pl.tau_off;
@@ -304,7 +314,7 @@ hipError_t time_class(int cls, int m, int n, int nrhs, int count, int dev, float
         e = hipMemcpyAsync(da, ha.data(), ha.size() * sizeof(T), hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
             e = hipMemcpyAsync(db, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = cbk::launch_clear_u32(reinterpret_cast<unsigned *>(dw), kBatchClasses, s);
+        if (e == hipSuccess) e = cbk::launch_clear_u32(reinterpret_cast<unsigned *>(dw), kBatchKernels, s);
         if (e == hipSuccess) e = hipEventRecord(e0, s);
         if (e == hipSuccess) e = launch_class<T>(&pl, cls, blocks, da, db, di, dw, s);
         if (e == hipSuccess) e = hipEventRecord(e1, s);
@@ -356,7 +366,7 @@ int cbk_gels_batch_plan(char prec, char trans, int nprob, const cbk_gels_problem
     std::vector<int> cls(nprob);
     std::vector<int> order(nprob);
     for (int k = 0; k < nprob; ++k) {
-        cls[k] = classify(desc[k], pl->tran, pl->tune);
+        cls[k] = kernel_of(classify(desc[k], pl->tran, pl->tune), desc[k]);
         order[k] = k;
     }
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -375,7 +385,7 @@ int cbk_gels_batch_plan(char prec, char trans, int nprob, const cbk_gels_problem
         }
         pl->begin[cls[order[i]] + 1]++;
     }
-    for (int c = 0; c < kBatchClasses; ++c)
+    for (int c = 0; c < kBatchKernels; ++c)
         pl->begin[c + 1] += pl->begin[c];
     pl->tau_off = pl->table_off + ((nprob * sizeof(BatchProblem) + 255) / 256) * 256;
     pl->bytes = pl->tau_off + (size_t)pl->tau_elems * esz;
@@ -388,8 +398,9 @@ size_t cbk_batch_plan_worksize(cbk_batch_plan_t plan) { return plan ? plan->byte
 int cbk_batch_plan_classes(cbk_batch_plan_t plan, int *count)
 {
     if (!plan || !count) return -1;
-    for (int c = 0; c < kBatchClasses; ++c)
-        count[c] = plan->begin[c + 1] - plan->begin[c];
+    for (int c = 0; c < kBatchClasses; ++c) count[c] = 0;
+    for (int k = 0; k < kBatchKernels; ++k)
+        count[class_of_kernel(k)] += plan->begin[k + 1] - plan->begin[k];
     return kBatchClasses;
 }
 
@@ -453,7 +464,8 @@ int cbk_autotune(void)
             const int nrhs = 2, n = w - nrhs, m = rows[b];
             float tn = 0.f, tw = 0.f;
             e = time_class<double>(b, m, n, nrhs, 16384, dev, &tn);
-            if (e == hipSuccess) e = time_class<double>(m <= 32 ? 3 : 4, m, n, nrhs, 16384, dev, &tw);
+            /* the wavefront kernel such a problem would take: two per wavefront */
+            if (e == hipSuccess) e = time_class<double>(m <= 32 ? 7 : 8, m, n, nrhs, 16384, dev, &tw);
             if (e != hipSuccess) return cbk::launch_status(e);
             if (tn > tw) break;
             maxw = w;

@@ -161,6 +161,9 @@ template <class T> struct BatchArgs {
  * lane kernel */
 constexpr int kBatchClasses = 7;
 constexpr int kBatchGeneral = 6;
+/* kernels behind the classes: 0-6 as the classes, and 7 / 8 the wavefront
+ * classes 3 / 4's problems with n + nrhs <= 32, two per wavefront */
+constexpr int kBatchKernels = 9;
 template <class T>
 hipError_t launch_gels_batch(int cls, const BatchArgs<T> &args, unsigned blocks, hipStream_t s);
 template <class T>

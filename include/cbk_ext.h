@@ -160,8 +160,8 @@ size_t cbk_batch_plan_worksize(cbk_batch_plan_t plan);
 
 /* problems per kernel class (count[CBK_BATCH_NCLASS]); returns the number
  * of classes. Classes: 0-2 one problem per 16-lane team (n + nrhs <= 16;
- * rows <= 16, 32, 64), 3-4 one problem per wavefront (n + nrhs <= 64; rows
- * <= 32, 64), 5 one problem per wavefront with A and B in separate registers
+ * rows <= 16, 32, 64), 3-4 the wavefront kernels (n + nrhs <= 64; rows
+ * <= 32, 64; two problems per wavefront when n + nrhs <= 32), 5 one problem per wavefront with A and B in separate registers
  * (n + nrhs > 64, rows <= 64, nrhs <= 64), 6 the general one-lane-per-problem
  * kernel (transposed or underdetermined problems, more than 64 rows, empty
  * problems). */

@@ -2,7 +2,8 @@
 oracle's batch gels on the same pools.
 
 Every kernel class is exercised: the team kernel (n + nrhs <= 16; rows <= 16,
-32, 64), the wavefront kernel (n + nrhs <= 64; rows <= 32, 64), the
+32, 64), the wavefront kernel (n + nrhs <= 64; rows <= 32, 64; two problems per
+wavefront when n + nrhs <= 32, one otherwise), the
 two-array wavefront kernel (n + nrhs > 64) and the general lane kernel
 (transposed, underdetermined, more than 64 rows, empty problems). The general kernel performs the oracle's
 operations and is compared bit for bit; the register kernels sum in another
@@ -47,6 +48,7 @@ def _class_mix(r):
         (64, 8, 8), (50, 13, 3), (64, 15, 1),                    # team, 64 rows
         (32, 30, 2), (24, 20, 8), (32, 32, 8),                   # wavefront, 32 rows
         (64, 32, 2), (64, 56, 8), (40, 33, 5), (63, 60, 4),      # wavefront, 64 rows
+        (64, 20, 4), (50, 25, 7), (33, 17, 1), (30, 9, 8),       # two per wavefront (n + nrhs <= 32)
         (64, 64, 1), (64, 60, 8), (60, 58, 7), (64, 57, 8),      # two-array wavefront
         (70, 10, 2), (30, 40, 2), (5, 0, 2), (6, 3, 0),          # general
     ] + batchcase.uneven_shapes(r, 300)
