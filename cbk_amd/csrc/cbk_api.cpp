@@ -92,6 +92,8 @@ int ormqr(char layout, char side, char trans, int m, int n, int k, const T *ap, 
      * (src/cbk_ormqr_compact.hpp:144) */
     const Operand<T> A = operand(const_cast<T *>(ap), ldap, row, nq, k, V);
     const Operand<T> C = operand(cp, ldcp, row, m, n, V);
+    if (current_policy() != CBK_POLICY_LANE && ormqr_reg_supports(nq, sizeof(T) == 8))
+        return status(launch_ormqr_reg<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
     if (current_policy() != CBK_POLICY_LANE && V >= 4 && ormqr_team_supports(nq))
         return status(launch_ormqr_team<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
     return status(launch_ormqr_lane<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));

@@ -85,6 +85,13 @@ template <class T>
 hipError_t launch_trsm_team(bool up, bool tran, bool unit, int m, int n, T alpha, Operand<T> A,
                             Operand<T> B, int V, int nm);
 
+/* ormqr with one lane per matrix and C in registers, reflector length nq (m
+ * for side L, n for side R) in {8, 16, 24}, and 32 in FP32 (ormqr_reg.hip) */
+bool ormqr_reg_supports(int nq, bool fp64);
+template <class T>
+hipError_t launch_ormqr_reg(bool left, bool tran, int m, int n, int k, Operand<T> A,
+                            const T *tau, Operand<T> C, int V, int nm);
+
 /* ormqr with reflector length nq (m for side L, n for side R) in {16, 32} */
 bool ormqr_team_supports(int nq);
 template <class T>
