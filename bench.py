@@ -296,7 +296,9 @@ def configs_bench(torch):
     rec("K2_dgeqrf_16x8", t, nm, 8 * (2 * m * n + n), "dgeqrf_compact 16x8")
     t = timeit(lambda: cbk_amd.call("d", "ormqr", "C", "L", "T", m, q, n, A, m, T, B, m, V, nm),
                lambda: B.copy_(B0))
-    rec("K2_dormqr_16x2_k8", t, nm, 8 * (m * n + n + 2 * m * q), "dormqr_compact L T, C 16x2, k=8")
+    # the reflector bodies below the diagonal are read, tau, and C read and written
+    rec("K2_dormqr_16x2_k8", t, nm, 8 * (m * n - n * (n + 1) // 2 + n + 2 * m * q),
+        "dormqr_compact L T, C 16x2, k=8")
     del A0, B0, A, B, T
 
     nm, n, q = 262144, 24, 24
