@@ -106,6 +106,14 @@ hipError_t launch_mixed_posv_team(bool up, int n, int nrhs, Operand<double> A,
                                   Operand<double> B, Operand<double> X, int V, int nm,
                                   int itermax, int *iter, int *info);
 
+/* QR / overdetermined least squares with one lane per matrix and the tall view
+ * F (p x q) in registers (qr_reg.hip): p x q in {8x4, 8x8, 16x4, 16x8, 32x4},
+ * and 32x8 in FP32; any V, any right-hand side count */
+bool qr_reg_supports(int p, int q, bool fp64);
+template <class T>
+hipError_t launch_qr_reg(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
+                         Operand<T> B, int V, int nm, int *info);
+
 /* QR / overdetermined least squares with the columns of [F | B] in the 16
  * lanes of a team (qr_cols.hip): p <= 32, q + nrhs <= 16, V >= 4 */
 bool qr_cols_supports(int p, int q, int nrhs);

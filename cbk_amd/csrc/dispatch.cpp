@@ -33,8 +33,11 @@ hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operan
     const bool tall = m >= n;
     const int p = tall ? m : n, q = tall ? n : m;
     const bool overdet = (tall != tran); /* src/cbk_gels_compact.hpp:124 */
-    /* the row team first at the orders it is tuned for (its butterfly sums
-     * measured faster there), the column team for the other small shapes */
+    /* F in one lane's registers where it fits; then the row team at the
+     * orders it is tuned for (its butterfly sums measured faster there), the
+     * column team for the other small shapes */
+    if (current_policy() != CBK_POLICY_LANE && overdet && qr_reg_supports(p, q, sizeof(T) == 8))
+        return launch_qr_reg<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
     if (current_policy() == CBK_POLICY_AUTO && overdet && V >= 4 &&
         qr_team_supports(p, q, nrhs, sizeof(T) == 8))
         return launch_qr_team<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
@@ -48,6 +51,8 @@ hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operan
 template <class T> hipError_t dispatch_geqrf(int m, int n, Operand<T> A, T *tau, int V, int nm)
 {
     const Operand<T> none = {nullptr, 1, A.row, 0};
+    if (current_policy() != CBK_POLICY_LANE && m >= n && qr_reg_supports(m, n, sizeof(T) == 8))
+        return launch_qr_reg<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
     if (current_policy() == CBK_POLICY_AUTO && m >= n && V >= 4 &&
         qr_team_supports(m, n, 0, sizeof(T) == 8))
         return launch_qr_team<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);

@@ -1,0 +1,255 @@
+/* qr_reg.hip -- geqrf_compact and the overdetermined gels_compact with one lane
+ * per matrix and the whole tall view F (p x q, F = A or A^T) in registers.
+ *
+ * The lane kernels (lane_kernels.hip) keep F in HBM and read and write it once
+ * per reflector; the team kernels spread a matrix over 16 lanes and are bound
+ * by the latency of their cross-lane sums. Here a lane loads F once, factors
+ * it in registers, stores it once, and then takes B one column at a time
+ * (load, apply the q reflectors, back substitution, store), so F, tau and B
+ * cross HBM once each; at V = 64 a wavefront's 64 lanes read one 512-byte run
+ * per element. Every register index is a compile-time constant (p and q are
+ * the template's P and Q exactly); the right-hand side count is a loop bound.
+ *
+ * The operations and their order are the lane kernel's, hence the
+ * reference's, so every output is bit-identical to it:
+ *   geqrf  larfg + larf per column (src/cbk_geqrf_compact.hpp:74-131):
+ *          tail = sum_{i>kk} F(i,kk)^2 (i ascending), norm = sqrt(x0^2 + tail),
+ *          beta = -sign(x0) norm, tau = (beta - x0)/beta, the body scaled by
+ *          1/(x0 - beta); each later column p: w = F(kk,p) + sum V(i) F(i,p),
+ *          w *= tau, F(kk,p) -= w, F(i,p) -= V(i) w.
+ *   gels   (src/cbk_gels_compact.hpp:84-146): Q^T B by the same larf per
+ *          column, reflectors forward (src/cbk_ormqr_compact.hpp:109-124),
+ *          then R X = B's top q rows by trsm_compact_group's form
+ *          (src/cbk_trsm_compact.hpp:73-145, 239-254): the dot form (x_i = (b_i
+ *          - sum_{l>i} R(i,l) x_l) / R(i,i), l ascending) for every column,
+ *          except the last one of an odd count when F and B are both unit
+ *          row stride, which takes the axpy form (x_k = b_k / R(k,k), then
+ *          b_i -= R(i,k) x_k for i < k). With info, an exact zero on R's
+ *          diagonal leaves B as Q^T B and reports its index + 1 (LAPACK
+ *          dgels's dtrtrs test), as the lane kernel does.
+ */
+
+#include "cbk_internal.h"
+#include "team_common.h"
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+namespace cbk {
+namespace {
+
+template <int B, int E, class F> __device__ __forceinline__ void rfor(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        rfor<B + 1, E>(f);
+    }
+}
+template <int B, int E, class F> __device__ __forceinline__ void rfor_down(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, E - 1>{});
+        rfor_down<B, E - 1>(f);
+    }
+}
+
+__device__ __forceinline__ double sqrtR(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float sqrtR(float x) { return __builtin_sqrtf(x); }
+
+constexpr int kBlock = 256;
+
+template <class T> struct RegQrArgs {
+    T *a;
+    int a_si, a_sj; /* element strides of the tall view F */
+    size_t a_gs;
+    T *tau;
+    T *b; /* null: geqrf */
+    int b_si, b_sj;
+    size_t b_gs;
+    int nrhs;
+    bool axpy_last; /* the last column takes trsm's axpy form */
+    int V;
+    long nlanes;
+    int nm;
+    int *info;
+};
+
+template <class T, int P, int Q>
+__global__ __launch_bounds__(kBlock) void qr_reg_kernel(RegQrArgs<T> p)
+{
+    constexpr unsigned SZ = sizeof(T);
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    /* the wavefront's first group (wave-uniform) bases the buffer resources;
+     * a lane's slot is a byte offset in them, an element's place a scalar one */
+    const long w0 = __builtin_amdgcn_readfirstlane(
+        (int)(((long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) / p.V));
+    if (idx >= p.nlanes) return;
+    const long g = idx / p.V;
+    const int v = (int)(idx % p.V);
+    const long groups_left = p.nlanes / p.V - w0;
+    const unsigned VS = (unsigned)p.V * SZ;
+    auto opaque = [](unsigned x) {
+        asm volatile("" : "+s"(x));
+        return x;
+    };
+    const __amdgpu_buffer_rsrc_t ra =
+        team::group_rsrc(p.a + w0 * p.a_gs, team::group_range(groups_left, p.a_gs, SZ));
+    const __amdgpu_buffer_rsrc_t rt = team::group_rsrc(
+        p.tau + w0 * (size_t)Q * p.V, team::group_range(groups_left, (size_t)Q * p.V, SZ));
+    const unsigned av = (unsigned)(((g - w0) * p.a_gs + v) * SZ);
+    const unsigned tv = (unsigned)(((g - w0) * (long)Q * p.V + v) * SZ);
+    const unsigned asiV = (unsigned)p.a_si * VS, asjV = (unsigned)p.a_sj * VS;
+
+    T f[P][Q];
+    {
+        const unsigned si = opaque(asiV), sj = opaque(asjV);
+        rfor<0, P>([&](auto i) {
+            rfor<0, Q>([&](auto j) { f[i][j] = team::bload(T(), ra, av, i * si + j * sj); });
+        });
+    }
+    /* geqrf (src/cbk_geqrf_compact.hpp:74-131) */
+    T tau[Q];
+    rfor<0, Q>([&](auto kc) {
+        constexpr int kk = kc;
+        T tail = T(0);
+        rfor<kk + 1, P>([&](auto i) { tail = team::fmaT(f[i][kk], f[i][kk], tail); });
+        const T x0 = f[kk][kk];
+        const T norm = sqrtR(team::fmaT(x0, x0, tail));
+        const T beta = (x0 >= T(0)) ? -norm : norm;
+        const bool has = tail > T(0);
+        const T t = has ? (beta - x0) / beta : T(0);
+        const T inv = has ? T(1) / (x0 - beta) : T(0);
+        f[kk][kk] = has ? beta : x0;
+        tau[kk] = t;
+        rfor<kk + 1, P>([&](auto i) { f[i][kk] = f[i][kk] * inv; });
+        rfor<kk + 1, Q>([&](auto c) {
+            T w = f[kk][c];
+            rfor<kk + 1, P>([&](auto i) { w = team::fmaT(f[i][kk], f[i][c], w); });
+            w = w * t;
+            f[kk][c] = f[kk][c] - w;
+            rfor<kk + 1, P>([&](auto i) { f[i][c] = team::fmaT(-f[i][kk], w, f[i][c]); });
+        });
+    });
+    {
+        const unsigned si = opaque(asiV), sj = opaque(asjV), ts = opaque(VS);
+        rfor<0, P>([&](auto i) {
+            rfor<0, Q>([&](auto j) { team::bstore(f[i][j], ra, av, i * si + j * sj); });
+        });
+        rfor<0, Q>([&](auto j) { team::bstore(tau[j], rt, tv, j * ts); });
+    }
+    if (!p.b) return;
+
+    /* info (real matrices only): the first exact zero on R's diagonal */
+    int bad = 0;
+    if (p.info && idx < p.nm) {
+        rfor_down<0, Q>([&](auto i) {
+            if (f[i][i] == T(0)) bad = i + 1;
+        });
+        p.info[idx] = bad;
+    }
+    const __amdgpu_buffer_rsrc_t rb =
+        team::group_rsrc(p.b + w0 * p.b_gs, team::group_range(groups_left, p.b_gs, SZ));
+    const unsigned bv = (unsigned)(((g - w0) * p.b_gs + v) * SZ);
+    const unsigned bsiV = (unsigned)p.b_si * VS, bsjV = (unsigned)p.b_sj * VS;
+    for (int j = 0; j < p.nrhs; ++j) {
+        const unsigned si = opaque(bsiV), col = (unsigned)j * opaque(bsjV);
+        T b[P];
+        rfor<0, P>([&](auto i) { b[i] = team::bload(T(), rb, bv, col + i * si); });
+        /* Q^T b, reflectors forward (src/cbk_ormqr_compact.hpp:68-124) */
+        rfor<0, Q>([&](auto kc) {
+            constexpr int kk = kc;
+            T w = b[kk];
+            rfor<kk + 1, P>([&](auto i) { w = team::fmaT(f[i][kk], b[i], w); });
+            w = w * tau[kk];
+            b[kk] = b[kk] - w;
+            rfor<kk + 1, P>([&](auto i) { b[i] = team::fmaT(-f[i][kk], w, b[i]); });
+        });
+        if (!bad) {
+            if (p.axpy_last && j == p.nrhs - 1) {
+                /* trsm_axpy_col (src/cbk_trsm_compact.hpp:49-67) */
+                rfor_down<0, Q>([&](auto kc) {
+                    constexpr int kk = kc;
+                    const T xk = b[kk] / f[kk][kk];
+                    b[kk] = xk;
+                    rfor<0, kk>([&](auto i) { b[i] = team::fmaT(-f[i][kk], xk, b[i]); });
+                });
+            }
+            else {
+                /* trsm_dot_block, one column (src/cbk_trsm_compact.hpp:73-104) */
+                rfor_down<0, Q>([&](auto ic) {
+                    constexpr int i = ic;
+                    T w = b[i];
+                    rfor<i + 1, Q>([&](auto l) { w = team::fmaT(-f[i][l], b[l], w); });
+                    b[i] = w / f[i][i];
+                });
+            }
+        }
+        rfor<0, P>([&](auto i) { team::bstore(b[i], rb, bv, col + i * si); });
+    }
+}
+
+template <class T, int P, int Q> hipError_t launch(const RegQrArgs<T> &a)
+{
+    const unsigned nb = (unsigned)((a.nlanes + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((qr_reg_kernel<T, P, Q>), dim3(nb), dim3(kBlock), 0, current_stream(), a);
+    return hipGetLastError();
+}
+
+} /* namespace */
+
+/* the shapes instantiated: F's p x q fits a lane's registers without spills */
+bool qr_reg_supports(int p, int q, bool fp64)
+{
+    if (p == 16 && q == 8) return true;
+    if (p == 8 && (q == 4 || q == 8)) return true;
+    if (p == 16 && q == 4) return true;
+    if (p == 32 && q == 4) return true;
+    return !fp64 && p == 32 && q == 8;
+}
+
+template <class T>
+hipError_t launch_qr_reg(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
+                         Operand<T> B, int V, int nm, int *info)
+{
+    RegQrArgs<T> a;
+    a.a = A.p;
+    int si = A.row ? A.ld : 1, sj = A.row ? 1 : A.ld;
+    if (transposed) {
+        const int tmp = si;
+        si = sj;
+        sj = tmp;
+    }
+    a.a_si = si;
+    a.a_sj = sj;
+    a.a_gs = A.gstride;
+    a.tau = tau;
+    a.b = B.p;
+    a.b_si = B.row ? B.ld : 1;
+    a.b_sj = B.row ? 1 : B.ld;
+    a.b_gs = B.gstride;
+    a.nrhs = B.p ? nrhs : 0;
+    /* trsm_compact_group's first branch (unit row strides of both views)
+     * takes the axpy form for the last column of an odd count */
+    a.axpy_last = (a.b_si == 1 && a.a_si == 1 && (nrhs % 4 == 1 || nrhs % 4 == 3));
+    a.V = V;
+    a.nlanes = (long)((nm + V - 1) / V) * V;
+    a.nm = nm;
+    a.info = B.p ? info : nullptr;
+    if (p == 16 && q == 8) return launch<T, 16, 8>(a);
+    if (p == 8 && q == 4) return launch<T, 8, 4>(a);
+    if (p == 8 && q == 8) return launch<T, 8, 8>(a);
+    if (p == 16 && q == 4) return launch<T, 16, 4>(a);
+    if (p == 32 && q == 4) return launch<T, 32, 4>(a);
+    if constexpr (sizeof(T) == 4) {
+        if (p == 32 && q == 8) return launch<T, 32, 8>(a);
+    }
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_qr_reg<double>(int, int, int, bool, Operand<double>, double *,
+                                          Operand<double>, int, int, int *);
+template hipError_t launch_qr_reg<float>(int, int, int, bool, Operand<float>, float *,
+                                         Operand<float>, int, int, int *);
+
+} /* namespace cbk */

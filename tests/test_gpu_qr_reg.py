@@ -1,0 +1,78 @@
+"""GPU parity of the register QR kernel (qr_reg.hip: geqrf and the
+overdetermined gels with one lane per matrix and the tall view in registers,
+p x q in {8x4, 8x8, 16x4, 16x8, 32x4}, and 32x8 in FP32) against the oracle,
+bit for bit: it performs the lane kernel's operations in the same order, so
+every output buffer (factor, tau, solution and residual rows, the leading-
+dimension padding and the padded group slots) must equal the oracle's exactly.
+Both layouts, transposed (wide A, trans T) and plain problems, odd and even
+right-hand side counts (the last column of an odd count takes trsm's axpy form
+where both views have unit row stride), V from 2 to 64 with partial last
+groups, and the info path."""
+
+import numpy as np
+import pytest
+
+import cases
+import cbkt
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu():
+    import gpuutil
+    return gpuutil
+
+
+def _check(case):
+    so, outo = cases.run(cbkt.oracle().call, case)
+    sg, outg = cases.run(_gpu().gpu_call, case)
+    assert so == sg == 0
+    for a, b in zip(outo, outg):
+        assert cases.same_bits(a, b), (case["routine"], case["p"], case["opts"], case["dims"],
+                                       case["V"], case["nm"])
+
+
+SHAPES = [("d", 16, 8), ("s", 16, 8), ("d", 8, 4), ("s", 8, 8), ("d", 8, 8), ("d", 16, 4),
+          ("s", 16, 4), ("d", 32, 4), ("s", 32, 4), ("s", 32, 8), ("s", 8, 4)]
+
+
+@pytest.mark.parametrize("p,m,n", SHAPES)
+def test_qr_reg_gels(p, m, n):
+    for i, (layout, trans, nrhs, V, nm) in enumerate([("C", "N", 2, 64, 130), ("C", "N", 3, 8, 13),
+                                                      ("R", "N", 1, 4, 9), ("C", "T", 5, 2, 5),
+                                                      ("R", "T", 2, 64, 70), ("C", "N", 1, 2, 3),
+                                                      ("R", "T", 1, 8, 11)]):
+        am, an = (m, n) if trans == "N" else (n, m)
+        print(f"case gels{p} {am}x{an} nrhs={nrhs} {layout}{trans} V={V} nm={nm}", flush=True)
+        _check(cases.make_case("gels", p, {"layout": layout, "trans": trans},
+                               {"m": am, "n": an, "nrhs": nrhs}, V, nm, seed=8100 + 31 * m + n + i,
+                               ld_extra=i % 2))
+
+
+@pytest.mark.parametrize("p,m,n", SHAPES)
+def test_qr_reg_geqrf(p, m, n):
+    for i, (layout, V, nm) in enumerate([("C", 64, 130), ("R", 8, 11), ("C", 2, 3)]):
+        print(f"case geqrf{p} {m}x{n} {layout} V={V} nm={nm}", flush=True)
+        _check(cases.make_case("geqrf", p, {"layout": layout}, {"m": m, "n": n}, V, nm,
+                               seed=8300 + 31 * m + n + i, ld_extra=1 - i % 2))
+
+
+def test_qr_reg_gels_info():
+    """exact zero columns: info = their index + 1, B left as Q^T B, the other
+    matrices solved normally; bit for bit with the oracle (LAPACK dgels)"""
+    V, nm, m, n, nrhs = 16, 40, 16, 8, 3
+    c = cases.make_case("gels", "d", {"layout": "C", "trans": "N"}, {"m": m, "n": n, "nrhs": nrhs},
+                        V, nm, seed=61)
+    A = cbkt.unpack(c["args"][5], nm, m, n, V)
+    A[3, :, 7] = 0.0
+    A[20, :, 0] = 0.0
+    c["args"][5] = cbkt.pack(A, V)
+    info_o = np.zeros(nm, np.int32)
+    info_g = np.zeros(nm, np.int32)
+    so, outo = cases.run(cbkt.oracle().call, c, info=info_o)
+    sg, outg = cases.run(_gpu().gpu_call, c, info=info_g)
+    assert so == sg == 0
+    assert info_o[3] == 8 and info_o[20] == 1 and np.count_nonzero(info_o) == 2
+    assert np.array_equal(info_o, info_g)
+    for a, b in zip(outo, outg):
+        assert cases.same_bits(a, b)
