@@ -112,7 +112,10 @@ int orgqr(char layout, int m, int n, int k, T *ap, int ldap, const T *taup, int 
     if (!vlen_ok(V)) return -8;
     if (nm < 0) return -9;
     if (m == 0 || n == 0 || nm == 0) return 0; /* k == 0 is not empty */
-    return status(launch_orgqr_lane<T>(m, n, k, operand(ap, ldap, row, m, n, V), taup, V, nm));
+    const Operand<T> A = operand(ap, ldap, row, m, n, V);
+    if (current_policy() != CBK_POLICY_LANE && qr_reg_supports(m, n, sizeof(T) == 8))
+        return status(launch_orgqr_reg<T>(m, n, k, A, taup, V, nm));
+    return status(launch_orgqr_lane<T>(m, n, k, A, taup, V, nm));
 }
 
 /* src/cbk.cpp:129-167: potrf and sytrfnp share their codes */

@@ -113,6 +113,9 @@ bool qr_reg_supports(int p, int q, bool fp64);
 template <class T>
 hipError_t launch_qr_reg(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
                          Operand<T> B, int V, int nm, int *info);
+/* orgqr of the same shapes (m x n = p x q), any k <= n */
+template <class T>
+hipError_t launch_orgqr_reg(int m, int n, int k, Operand<T> A, const T *tau, int V, int nm);
 
 /* QR / overdetermined least squares with the columns of [F | B] in the 16
  * lanes of a team (qr_cols.hip): p <= 32, q + nrhs <= 16, V >= 4 */

@@ -1,5 +1,6 @@
-/* qr_reg.hip -- geqrf_compact and the overdetermined gels_compact with one lane
- * per matrix and the whole tall view F (p x q, F = A or A^T) in registers.
+/* qr_reg.hip -- geqrf_compact, the overdetermined gels_compact and
+ * orgqr_compact with one lane per matrix and the whole tall view F (p x q,
+ * F = A or A^T; Q for orgqr) in registers.
  *
  * The lane kernels (lane_kernels.hip) keep F in HBM and read and write it once
  * per reflector; the team kernels spread a matrix over 16 lanes and are bound
@@ -196,6 +197,81 @@ template <class T, int P, int Q> hipError_t launch(const RegQrArgs<T> &a)
     return hipGetLastError();
 }
 
+
+/* orgqr_compact_group (src/cbk_orgqr_compact.hpp:51-85) with Q (p x q) in
+ * registers: the columns j >= k start as e_j, then for kk = k-1 down to 0 the
+ * reflector kk is applied to the columns right of it by the same larf and its
+ * own column becomes -tau v below the diagonal, 1 - tau on it and 0 above */
+template <class T, int P, int Q>
+__global__ __launch_bounds__(kBlock) void orgqr_reg_kernel(int k, Operand<T> A, const T *tau_p,
+                                                           int V, long nlanes)
+{
+    constexpr unsigned SZ = sizeof(T);
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long w0 = __builtin_amdgcn_readfirstlane(
+        (int)(((long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) / V));
+    if (idx >= nlanes) return;
+    const long g = idx / V;
+    const int v = (int)(idx % V);
+    const long groups_left = nlanes / V - w0;
+    const unsigned VS = (unsigned)V * SZ;
+    auto opaque = [](unsigned x) {
+        asm volatile("" : "+s"(x));
+        return x;
+    };
+    const __amdgpu_buffer_rsrc_t ra =
+        team::group_rsrc(A.p + w0 * A.gstride, team::group_range(groups_left, A.gstride, SZ));
+    const __amdgpu_buffer_rsrc_t rt = team::group_rsrc(
+        tau_p + w0 * (size_t)k * V, team::group_range(groups_left, (size_t)k * V, SZ));
+    const unsigned av = (unsigned)(((g - w0) * A.gstride + v) * SZ);
+    const unsigned tv = (unsigned)(((g - w0) * (long)k * V + v) * SZ);
+    const unsigned asiV = (unsigned)(A.row ? A.ld : 1) * VS, asjV = (unsigned)(A.row ? 1 : A.ld) * VS;
+
+    T f[P][Q];
+    {
+        const unsigned si = opaque(asiV), sj = opaque(asjV);
+        rfor<0, P>([&](auto i) {
+            rfor<0, Q>([&](auto j) {
+                /* only the reflector bodies (columns < k) are read */
+                f[i][j] = team::bload(T(), ra, j < k ? av : team::kOOB, i * si + j * sj);
+                if (j >= k) f[i][j] = (i == j) ? T(1) : T(0);
+            });
+        });
+    }
+    rfor_down<0, Q>([&](auto kc) {
+        constexpr int kk = kc;
+        if (kk < k) {
+            const T t = team::bload(T(), rt, tv, (unsigned)kk * opaque(VS));
+            rfor<kk + 1, Q>([&](auto c) {
+                T w = f[kk][c];
+                rfor<kk + 1, P>([&](auto i) { w = team::fmaT(f[i][kk], f[i][c], w); });
+                w = w * t;
+                f[kk][c] = f[kk][c] - w;
+                rfor<kk + 1, P>([&](auto i) { f[i][c] = team::fmaT(-f[i][kk], w, f[i][c]); });
+            });
+            const T mt = -t;
+            rfor<kk + 1, P>([&](auto i) { f[i][kk] = f[i][kk] * mt; });
+            f[kk][kk] = T(1) - t;
+            rfor<0, kk>([&](auto i) { f[i][kk] = T(0); });
+        }
+    });
+    {
+        const unsigned si = opaque(asiV), sj = opaque(asjV);
+        rfor<0, P>([&](auto i) {
+            rfor<0, Q>([&](auto j) { team::bstore(f[i][j], ra, av, i * si + j * sj); });
+        });
+    }
+}
+
+template <class T, int P, int Q>
+hipError_t launch_orgqr(int k, Operand<T> A, const T *tau, int V, long nl)
+{
+    const unsigned nb = (unsigned)((nl + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((orgqr_reg_kernel<T, P, Q>), dim3(nb), dim3(kBlock), 0, current_stream(),
+                       k, A, tau, V, nl);
+    return hipGetLastError();
+}
+
 } /* namespace */
 
 /* the shapes instantiated: F's p x q fits a lane's registers without spills */
@@ -251,5 +327,24 @@ template hipError_t launch_qr_reg<double>(int, int, int, bool, Operand<double>, 
                                           Operand<double>, int, int, int *);
 template hipError_t launch_qr_reg<float>(int, int, int, bool, Operand<float>, float *,
                                          Operand<float>, int, int, int *);
+
+template <class T>
+hipError_t launch_orgqr_reg(int m, int n, int k, Operand<T> A, const T *tau, int V, int nm)
+{
+    const long nl = (long)((nm + V - 1) / V) * V;
+    if (m == 16 && n == 8) return launch_orgqr<T, 16, 8>(k, A, tau, V, nl);
+    if (m == 8 && n == 4) return launch_orgqr<T, 8, 4>(k, A, tau, V, nl);
+    if (m == 8 && n == 8) return launch_orgqr<T, 8, 8>(k, A, tau, V, nl);
+    if (m == 16 && n == 4) return launch_orgqr<T, 16, 4>(k, A, tau, V, nl);
+    if (m == 32 && n == 4) return launch_orgqr<T, 32, 4>(k, A, tau, V, nl);
+    if constexpr (sizeof(T) == 4) {
+        if (m == 32 && n == 8) return launch_orgqr<T, 32, 8>(k, A, tau, V, nl);
+    }
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_orgqr_reg<double>(int, int, int, Operand<double>, const double *, int,
+                                             int);
+template hipError_t launch_orgqr_reg<float>(int, int, int, Operand<float>, const float *, int, int);
 
 } /* namespace cbk */

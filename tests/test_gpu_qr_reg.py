@@ -1,5 +1,5 @@
-"""GPU parity of the register QR kernel (qr_reg.hip: geqrf and the
-overdetermined gels with one lane per matrix and the tall view in registers,
+"""GPU parity of the register QR kernels (qr_reg.hip: geqrf, the
+overdetermined gels and orgqr with one lane per matrix and the tall view in registers,
 p x q in {8x4, 8x8, 16x4, 16x8, 32x4}, and 32x8 in FP32) against the oracle,
 bit for bit: it performs the lane kernel's operations in the same order, so
 every output buffer (factor, tau, solution and residual rows, the leading-
@@ -76,3 +76,13 @@ def test_qr_reg_gels_info():
     assert np.array_equal(info_o, info_g)
     for a, b in zip(outo, outg):
         assert cases.same_bits(a, b)
+
+
+@pytest.mark.parametrize("p,m,n", SHAPES)
+def test_qr_reg_orgqr(p, m, n):
+    """orgqr of the same shapes: all k from 0 to n over the cases, bit for bit"""
+    for i, (layout, k, V, nm) in enumerate([("C", n, 64, 130), ("R", n - 1, 8, 11), ("C", n // 2, 2, 3),
+                                            ("R", 0, 4, 9), ("C", 1, 16, 17)]):
+        print(f"case orgqr{p} {m}x{n} k={k} {layout} V={V} nm={nm}", flush=True)
+        _check(cases.make_case("orgqr", p, {"layout": layout}, {"m": m, "n": n, "k": k}, V, nm,
+                               seed=8500 + 31 * m + n + i, ld_extra=i % 2))
