@@ -64,9 +64,9 @@ hipError_t launch_mixed_posv_lane(bool up, int n, int nrhs, Operand<double> A,
 
 /* ---- team kernels: one matrix per 16-lane team, held in registers ---- */
 
-/* orders n <= 12: the register lane kernel (reg_chol.hip), same contract as
- * launch_chol_lane */
-bool reg_chol_supports(int n);
+/* orders n <= 16 in FP64, n <= 24 in FP32: the register lane kernel
+ * (reg_chol.hip), same contract as launch_chol_lane */
+bool reg_chol_supports(int n, bool fp64);
 template <class T>
 hipError_t launch_reg_chol(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
                            Operand<T> B, int V, int nm, int *info);

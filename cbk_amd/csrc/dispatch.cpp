@@ -10,10 +10,12 @@ template <class T>
 hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
                         Operand<T> B, int V, int nm, int *info)
 {
+    /* the whole triangle in one lane's registers where it fits (measured
+     * faster than the team kernel at its orders 16 and 24 too) */
+    if (current_policy() != CBK_POLICY_LANE && reg_chol_supports(n, sizeof(T) == 8))
+        return launch_reg_chol<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     if (current_policy() == CBK_POLICY_AUTO && sym_team_supports(n) && V >= 4)
         return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
-    if (current_policy() != CBK_POLICY_LANE && reg_chol_supports(n))
-        return launch_reg_chol<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
 }
 

@@ -1,6 +1,8 @@
 /* reg_chol.hip -- potrf/posv/potrs and sytrfnp/sysvnp/sytrsnp for small
- * orders (n <= 12) with one lane per matrix and the whole matrix in
- * registers.
+ * orders (n <= 16 in FP64, n <= 24 in FP32) with one lane per matrix and the
+ * whole lower triangle in registers (at the largest orders a lane holds up
+ * to 300 values: the compiler parks part of them in accumulation registers,
+ * one wavefront per SIMD, no scratch).
  *
  * The general lane kernel (lane_kernels.hip) works in place in HBM; at small n
  * the arithmetic per byte is low and every element access is a round trip. Here
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void reg_chol_kernel(bool factor, bool up, 
 
 } /* namespace */
 
-bool reg_chol_supports(int n) { return n >= 1 && n <= 12; }
+bool reg_chol_supports(int n, bool fp64) { return n >= 1 && n <= (fp64 ? 16 : 24); }
 
 template <class T>
 hipError_t launch_reg_chol(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
@@ -192,18 +194,19 @@ hipError_t launch_reg_chol(bool ldl, bool factor, bool up, int n, int nrhs, Oper
     const long nl = (long)((nm + V - 1) / V) * V;
     const dim3 grid((unsigned)((nl + kBlock - 1) / kBlock)), block(kBlock);
     const hipStream_t s = current_stream();
-    if (n <= 8) {
-        if (ldl)
-            hipLaunchKernelGGL((reg_chol_kernel<T, 8, true>), grid, block, 0, s, factor, up, n, nrhs, A, B, V, nl, nm, info);
-        else
-            hipLaunchKernelGGL((reg_chol_kernel<T, 8, false>), grid, block, 0, s, factor, up, n, nrhs, A, B, V, nl, nm, info);
-    }
-    else {
-        if (ldl)
-            hipLaunchKernelGGL((reg_chol_kernel<T, 12, true>), grid, block, 0, s, factor, up, n, nrhs, A, B, V, nl, nm, info);
-        else
-            hipLaunchKernelGGL((reg_chol_kernel<T, 12, false>), grid, block, 0, s, factor, up, n, nrhs, A, B, V, nl, nm, info);
-    }
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, block, 0, s, factor, up, n, nrhs, A, B, V, nl, nm, info);
+    };
+    if (n <= 8)
+        ldl ? go(reg_chol_kernel<T, 8, true>) : go(reg_chol_kernel<T, 8, false>);
+    else if (n <= 12)
+        ldl ? go(reg_chol_kernel<T, 12, true>) : go(reg_chol_kernel<T, 12, false>);
+    else if (n <= 16)
+        ldl ? go(reg_chol_kernel<T, 16, true>) : go(reg_chol_kernel<T, 16, false>);
+    else if constexpr (sizeof(T) == 4)
+        ldl ? go(reg_chol_kernel<T, 24, true>) : go(reg_chol_kernel<T, 24, false>);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

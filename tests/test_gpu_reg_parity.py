@@ -1,5 +1,6 @@
 """GPU parity of the register lane kernel (potrf/posv/potrs, sytrfnp/sysvnp/
-sytrsnp at n <= 12) against the oracle, bit for bit -- factors and solutions:
+sytrsnp at n <= 16 in FP64, n <= 24 in FP32) against the oracle, bit for bit
+-- factors and solutions:
 the kernel performs the lane kernel's operations in registers, including the
 reference trsm router's choice of the axpy form for a single leftover
 right-hand side of the backward sweep (nrhs = 1, 3, 5 with column-major B).
@@ -19,8 +20,8 @@ def _gpu():
     return gpuutil
 
 
-@pytest.mark.parametrize("p", ["d", "s"])
-@pytest.mark.parametrize("n", [1, 3, 8, 9, 12])
+@pytest.mark.parametrize("p,n", [(p, n) for p in "ds" for n in (1, 3, 8, 9, 12, 13, 16)] +
+                         [("s", 20), ("s", 24)])
 def test_reg_factor_and_solve_bitwise(p, n):
     for i, (lay, uplo) in enumerate((("C", "L"), ("C", "U"), ("R", "L"), ("R", "U"))):
         for V, nm in ((64, 130), (8, 13), (2, 5)):
