@@ -23,6 +23,8 @@ template <class T>
 hipError_t dispatch_trsm(bool left, bool up, bool tran, bool unit, int m, int n, T alpha,
                          Operand<T> A, Operand<T> B, int V, int nm)
 {
+    if (current_policy() != CBK_POLICY_LANE && trsm_reg_supports(left, m, sizeof(T) == 8))
+        return launch_trsm_reg<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
     if (current_policy() != CBK_POLICY_LANE && V >= 4 && trsm_team_supports(left, up, tran, m))
         return launch_trsm_team<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
     return launch_trsm_lane<T>(left, up, tran, unit, m, n, alpha, A, B, V, nm);
