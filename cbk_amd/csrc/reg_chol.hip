@@ -14,13 +14,16 @@
  * (the row-dot form, and the axpy form where the router picks it for a
  * single leftover column of the backward sweep; in the forward sweep the two
  * forms perform the same operations per element) -- on registers, and stores
- * the results once. Loops run to the
+ * the results once. Elements are addressed through buffer resources based at
+ * the wavefront's first group (a lane's slot a 32-bit offset, an element's
+ * place a scalar one); rows past n read zeros and drop their stores. Loops run to the
  * compile-time bound NMAX with wave-uniform guards on the run-time n, so
  * every register index is a constant. Results are bit-identical to the lane
  * kernel, hence to the reference.
  */
 
 #include "cbk_internal.h"
+#include "team_common.h"
 
 #include <hip/hip_runtime.h>
 
@@ -37,8 +40,6 @@ template <int B, int E, class F> __device__ __forceinline__ void rfor(F &&f)
     }
 }
 
-__device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double sqrtR(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float sqrtR(float x) { return __builtin_sqrtf(x); }
 
@@ -50,10 +51,19 @@ __global__ __launch_bounds__(kBlock) void reg_chol_kernel(bool factor, bool up, 
                                                           Operand<T> A, Operand<T> B, int V,
                                                           long nlanes, int nm, int *info)
 {
+    constexpr unsigned SZ = sizeof(T);
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long w0 = __builtin_amdgcn_readfirstlane(
+        (int)(((long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) / V));
     if (idx >= nlanes) return;
     const long g = idx / V;
     const int v = (int)(idx % V);
+    const long groups_left = nlanes / V - w0;
+    const unsigned VS = (unsigned)V * SZ;
+    auto opaque = [](unsigned x) {
+        asm volatile("" : "+s"(x));
+        return x;
+    };
     /* the lower view: uplo 'U' reads A transposed (src/cbk_common.hpp:201-206) */
     int si = A.row ? A.ld : 1, sj = A.row ? 1 : A.ld;
     if (up) {
@@ -61,13 +71,18 @@ __global__ __launch_bounds__(kBlock) void reg_chol_kernel(bool factor, bool up, 
         si = sj;
         sj = tmp;
     }
-    T *ap = A.p + (size_t)g * A.gstride + v;
-    auto aix = [&](int i, int j) { return (size_t)(i * si + j * sj) * V; };
+    const __amdgpu_buffer_rsrc_t ra =
+        team::group_rsrc(A.p + w0 * A.gstride, team::group_range(groups_left, A.gstride, SZ));
+    const unsigned av = (unsigned)(((g - w0) * A.gstride + v) * SZ);
+    const unsigned siV = (unsigned)si * VS, sjV = (unsigned)sj * VS;
 
     T a[NMAX][NMAX];
-    rfor<0, NMAX>([&](auto i) {
-        rfor<0, i + 1>([&](auto j) { a[i][j] = (i < n) ? ap[aix(i, j)] : T(0); });
-    });
+    {
+        const unsigned s_i = opaque(siV), s_j = opaque(sjV);
+        rfor<0, NMAX>([&](auto i) {
+            rfor<0, i + 1>([&](auto j) { a[i][j] = team::bload(T(), ra, i < n ? av : team::kOOB, i * s_i + j * s_j); });
+        });
+    }
 
     const bool real = idx < nm;
     int linfo = 0;
@@ -85,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void reg_chol_kernel(bool factor, bool up, 
                     if (c < pe) {
                         const T w = a[c][j] * inv;
                         rfor<c, NMAX>([&](auto i) {
-                            if (i < n) a[i][c] = fmaR(-a[i][j], w, a[i][c]);
+                            if (i < n) a[i][c] = team::fmaT(-a[i][j], w, a[i][c]);
                         });
                     }
                 });
@@ -105,17 +120,16 @@ __global__ __launch_bounds__(kBlock) void reg_chol_kernel(bool factor, bool up, 
                     if (c >= pe && c < n) {
                         const T w = LDL ? a[c][j] * a[j][j] : a[c][j];
                         rfor<c, NMAX>([&](auto i) {
-                            if (i < n) a[i][c] = fmaR(-a[i][j], w, a[i][c]);
+                            if (i < n) a[i][c] = team::fmaT(-a[i][j], w, a[i][c]);
                         });
                     }
                 });
             }
         });
         if (info && real) info[idx] = linfo;
+        const unsigned s_i = opaque(siV), s_j = opaque(sjV);
         rfor<0, NMAX>([&](auto i) {
-            rfor<0, i + 1>([&](auto j) {
-                if (i < n) ap[aix(i, j)] = a[i][j];
-            });
+            rfor<0, i + 1>([&](auto j) { team::bstore(a[i][j], ra, i < n ? av : team::kOOB, i * s_i + j * s_j); });
         });
     }
     if (nrhs <= 0 || linfo != 0 && info) return;
@@ -124,7 +138,10 @@ __global__ __launch_bounds__(kBlock) void reg_chol_kernel(bool factor, bool up, 
      * the reciprocal-then-multiply by D, then backward with L^T (the dot form
      * of the reference's trsm, src/cbk_trsm_compact.hpp:73-104) */
     const int bsi = B.row ? B.ld : 1, bsj = B.row ? 1 : B.ld;
-    T *bp = B.p + (size_t)g * B.gstride + v;
+    const __amdgpu_buffer_rsrc_t rb =
+        team::group_rsrc(B.p + w0 * B.gstride, team::group_range(groups_left, B.gstride, SZ));
+    const unsigned bv = (unsigned)(((g - w0) * B.gstride + v) * SZ);
+    const unsigned bsiV = (unsigned)bsi * VS, bsjV = (unsigned)bsj * VS;
     T invd[NMAX];
     if (LDL) {
         rfor<0, NMAX>([&](auto i) { invd[i] = (i < n) ? T(1) / a[i][i] : T(0); });
@@ -140,11 +157,12 @@ __global__ __launch_bounds__(kBlock) void reg_chol_kernel(bool factor, bool up, 
     const bool single_tail = (nrhs % 4 == 1) || (nrhs % 4 == 3);
     for (int q = 0; q < nrhs; ++q) {
         T b[NMAX];
-        rfor<0, NMAX>([&](auto i) { b[i] = (i < n) ? bp[(size_t)(i * bsi + q * bsj) * V] : T(0); });
+        const unsigned b_i = opaque(bsiV), col = (unsigned)q * opaque(bsjV);
+        rfor<0, NMAX>([&](auto i) { b[i] = team::bload(T(), rb, i < n ? bv : team::kOOB, col + i * b_i); });
         rfor<0, NMAX>([&](auto i) {
             if (i < n) {
                 T w = b[i] * T(1);
-                rfor<0, i>([&](auto l) { w = fmaR(-a[i][l], b[l], w); });
+                rfor<0, i>([&](auto l) { w = team::fmaT(-a[i][l], b[l], w); });
                 b[i] = LDL ? w : w / a[i][i];
             }
         });
@@ -161,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void reg_chol_kernel(bool factor, bool up, 
                 if (k < n) {
                     const T x = LDL ? b[k] : b[k] / a[k][k];
                     b[k] = x;
-                    rfor<0, k>([&](auto i) { b[i] = fmaR(-a[k][i], x, b[i]); });
+                    rfor<0, k>([&](auto i) { b[i] = team::fmaT(-a[k][i], x, b[i]); });
                 }
             });
         }
@@ -171,15 +189,13 @@ __global__ __launch_bounds__(kBlock) void reg_chol_kernel(bool factor, bool up, 
                 if (i < n) {
                     T w = b[i] * T(1);
                     rfor<i + 1, NMAX>([&](auto l) {
-                        if (l < n) w = fmaR(-a[l][i], b[l], w);
+                        if (l < n) w = team::fmaT(-a[l][i], b[l], w);
                     });
                     b[i] = LDL ? w : w / a[i][i];
                 }
             });
         }
-        rfor<0, NMAX>([&](auto i) {
-            if (i < n) bp[(size_t)(i * bsi + q * bsj) * V] = b[i];
-        });
+        rfor<0, NMAX>([&](auto i) { team::bstore(b[i], rb, i < n ? bv : team::kOOB, col + i * b_i); });
     }
 }
 
