@@ -80,7 +80,7 @@ hipError_t launch_sym_team(bool ldl, bool factor, bool up, int n, int nrhs, Oper
                            Operand<T> B, int V, int nm, int *info);
 
 /* trsm, side L, with one lane per matrix and op(A)'s triangle in registers
- * (trsm_reg.hip): m in {8, 16}, and 24 in FP32 */
+ * (trsm_reg.hip): m <= 16, and m <= 24 in FP32 */
 bool trsm_reg_supports(bool left, int m, bool fp64);
 template <class T>
 hipError_t launch_trsm_reg(bool up, bool tran, bool unit, int m, int n, T alpha, Operand<T> A,

@@ -1,5 +1,9 @@
 /* trsm_reg.hip -- trsm_compact, side L, with one lane per matrix and the
- * triangle of op(A) in registers, for m in {8, 16} (and 24 in FP32).
+ * triangle of op(A) in registers, for m <= 16 (and m <= 24 in FP32).
+ * Loops run to the compile-time bound M (8, 16 or 24) with wave-uniform
+ * guards on the run-time m, so every register index is a constant; rows past
+ * m are neither loaded, computed nor stored, and at m == M (EXACT) the guards
+ * fold away.
  *
  * The lane kernel (lane_kernels.hip) reads the triangle from HBM once per
  * right-hand side; the team kernel (trsm_team_impl.h) spreads a matrix over 16
@@ -49,6 +53,7 @@ template <class T> struct RegTrsmArgs {
     T *b;
     int b_si, b_sj;
     size_t b_gs;
+    int m;
     int n;
     T alpha;
     bool unit;
@@ -60,10 +65,11 @@ template <class T> struct RegTrsmArgs {
 /* index of the k-th row solved (pivot order) */
 template <bool BACK, int M, int K> constexpr int piv() { return BACK ? M - 1 - K : K; }
 
-template <class T, int M, bool BACK>
+template <class T, int M, bool BACK, bool EXACT>
 __global__ __launch_bounds__(kBlock) void trsm_reg_kernel(RegTrsmArgs<T> p)
 {
     constexpr unsigned SZ = sizeof(T);
+    const int m = EXACT ? M : p.m;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long w0 = __builtin_amdgcn_readfirstlane(
         (int)(((long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) / p.V));
@@ -90,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void trsm_reg_kernel(RegTrsmArgs<T> p)
         rfor<0, M>([&](auto i) {
             rfor<0, M>([&](auto l) {
                 if constexpr (BACK ? (l >= i) : (l <= i))
-                    e[i][l] = team::bload(T(), ra, (l == i && p.unit) ? team::kOOB : av,
+                    e[i][l] = team::bload(T(), ra, ((l == i && p.unit) || i >= m || l >= m) ? team::kOOB : av,
                                           i * si + l * sj);
             });
         });
@@ -100,15 +106,21 @@ __global__ __launch_bounds__(kBlock) void trsm_reg_kernel(RegTrsmArgs<T> p)
     for (int j = 0; j < p.n; ++j) {
         const unsigned si = opaque(bsiV), col = (unsigned)j * opaque(bsjV);
         T b[M];
-        rfor<0, M>([&](auto i) { b[i] = team::bload(T(), rb, bv, col + i * si); });
+        rfor<0, M>([&](auto i) { b[i] = team::bload(T(), rb, i < m ? bv : team::kOOB, col + i * si); });
         if (p.axpy_last && j == p.n - 1) {
             /* trsm_axpy_col (src/cbk_trsm_compact.hpp:49-67) */
-            rfor<0, M>([&](auto i) { b[i] = b[i] * alpha; });
+            rfor<0, M>([&](auto i) {
+                if (i < m) b[i] = b[i] * alpha;
+            });
             rfor<0, M>([&](auto kc) {
                 constexpr int k = piv<BACK, M, kc>();
-                const T xk = p.unit ? b[k] : b[k] / e[k][k];
-                b[k] = xk;
-                rfor<BACK ? 0 : k + 1, BACK ? k : M>([&](auto i) { b[i] = team::fmaT(-e[i][k], xk, b[i]); });
+                if (k < m) {
+                    const T xk = p.unit ? b[k] : b[k] / e[k][k];
+                    b[k] = xk;
+                    rfor<BACK ? 0 : k + 1, BACK ? k : M>([&](auto i) {
+                        if (i < m) b[i] = team::fmaT(-e[i][k], xk, b[i]);
+                    });
+                }
             });
         }
         else {
@@ -116,22 +128,31 @@ __global__ __launch_bounds__(kBlock) void trsm_reg_kernel(RegTrsmArgs<T> p)
              * (src/cbk_trsm_compact.hpp:73-104, 179-200) */
             rfor<0, M>([&](auto tc) {
                 constexpr int i = piv<BACK, M, tc>();
-                T w = b[i] * alpha;
-                rfor<BACK ? i + 1 : 0, BACK ? M : i>([&](auto l) { w = team::fmaT(-e[i][l], b[l], w); });
-                b[i] = p.unit ? w : w / e[i][i];
+                if (i < m) {
+                    T w = b[i] * alpha;
+                    rfor<BACK ? i + 1 : 0, BACK ? M : i>([&](auto l) {
+                        if (l < m) w = team::fmaT(-e[i][l], b[l], w);
+                    });
+                    b[i] = p.unit ? w : w / e[i][i];
+                }
             });
         }
-        rfor<0, M>([&](auto i) { team::bstore(b[i], rb, bv, col + i * si); });
+        rfor<0, M>([&](auto i) { team::bstore(b[i], rb, i < m ? bv : team::kOOB, col + i * si); });
     }
 }
 
 template <class T, int M> hipError_t launch(bool back, const RegTrsmArgs<T> &a)
 {
-    const unsigned nb = (unsigned)((a.nlanes + kBlock - 1) / kBlock);
-    if (back)
-        hipLaunchKernelGGL((trsm_reg_kernel<T, M, true>), dim3(nb), dim3(kBlock), 0, current_stream(), a);
-    else
-        hipLaunchKernelGGL((trsm_reg_kernel<T, M, false>), dim3(nb), dim3(kBlock), 0, current_stream(), a);
+    const dim3 grid((unsigned)((a.nlanes + kBlock - 1) / kBlock)), block(kBlock);
+    const hipStream_t s = current_stream();
+    if (a.m == M) {
+        if (back) hipLaunchKernelGGL((trsm_reg_kernel<T, M, true, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((trsm_reg_kernel<T, M, false, true>), grid, block, 0, s, a);
+    }
+    else {
+        if (back) hipLaunchKernelGGL((trsm_reg_kernel<T, M, true, false>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((trsm_reg_kernel<T, M, false, false>), grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -139,7 +160,7 @@ template <class T, int M> hipError_t launch(bool back, const RegTrsmArgs<T> &a)
 
 bool trsm_reg_supports(bool left, int m, bool fp64)
 {
-    return left && (m == 8 || m == 16 || (!fp64 && m == 24));
+    return left && m >= 1 && m <= (fp64 ? 16 : 24);
 }
 
 template <class T>
@@ -156,6 +177,7 @@ hipError_t launch_trsm_reg(bool up, bool tran, bool unit, int m, int n, T alpha,
     a.b_si = B.row ? B.ld : 1;
     a.b_sj = B.row ? 1 : B.ld;
     a.b_gs = B.gstride;
+    a.m = m;
     a.n = n;
     a.alpha = alpha;
     a.unit = unit;
@@ -168,10 +190,10 @@ hipError_t launch_trsm_reg(bool up, bool tran, bool unit, int m, int n, T alpha,
     a.V = V;
     a.nlanes = (long)((nm + V - 1) / V) * V;
     const bool back = (up != tran);
-    if (m == 8) return launch<T, 8>(back, a);
-    if (m == 16) return launch<T, 16>(back, a);
+    if (m <= 8) return launch<T, 8>(back, a);
+    if (m <= 16) return launch<T, 16>(back, a);
     if constexpr (sizeof(T) == 4) {
-        if (m == 24) return launch<T, 24>(back, a);
+        if (m <= 24) return launch<T, 24>(back, a);
     }
     return hipErrorInvalidValue;
 }

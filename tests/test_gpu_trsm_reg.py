@@ -1,5 +1,6 @@
-"""GPU parity of the register trsm kernel (trsm_reg.hip: side L, m in {8, 16},
-and 24 in FP32, one lane per matrix with op(A)'s triangle in registers)
+"""GPU parity of the register trsm kernel (trsm_reg.hip: side L, m <= 16, and
+m <= 24 in FP32, exact and guarded orders, one lane per matrix with op(A)'s
+triangle in registers)
 against the oracle, bit for bit: every uplo / trans / diag, both layouts,
 alpha 1 and not 1, odd and even right-hand side counts (the last column of an
 odd count takes the reference router's axpy form where B has unit row stride
@@ -21,7 +22,8 @@ def _gpu():
     return gpuutil
 
 
-@pytest.mark.parametrize("p,m", [("d", 8), ("s", 8), ("d", 16), ("s", 16), ("s", 24)])
+@pytest.mark.parametrize("p,m", [("d", 8), ("s", 8), ("d", 16), ("s", 16), ("s", 24), ("s", 1), ("d", 5),
+                                 ("s", 12), ("d", 13), ("s", 20)])
 def test_trsm_reg(p, m):
     combos = list(itertools.product("CR", "UL", "NT", "UN"))
     for i, (lay, uplo, tr, diag) in enumerate(combos):
