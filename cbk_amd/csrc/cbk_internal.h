@@ -114,8 +114,8 @@ hipError_t launch_mixed_posv_team(bool up, int n, int nrhs, Operand<double> A,
                                   int itermax, int *iter, int *info);
 
 /* QR / overdetermined least squares with one lane per matrix and the tall view
- * F (p x q) in registers (qr_reg.hip): p x q in {8x4, 8x8, 16x4, 16x8, 32x4},
- * and 32x8 in FP32; any V, any right-hand side count */
+ * F (p x q) in registers (qr_reg.hip): p <= 16 with q <= 8, p <= 32 with
+ * q <= 4 (q <= 8 in FP32); any V, any right-hand side count */
 bool qr_reg_supports(int p, int q, bool fp64);
 template <class T>
 hipError_t launch_qr_reg(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,

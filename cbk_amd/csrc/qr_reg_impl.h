@@ -1,6 +1,10 @@
-/* qr_reg.hip -- geqrf_compact, the overdetermined gels_compact and
+/* qr_reg_impl.h -- geqrf_compact, the overdetermined gels_compact and
  * orgqr_compact with one lane per matrix and the whole tall view F (p x q,
- * F = A or A^T; Q for orgqr) in registers.
+ * F = A or A^T; Q for orgqr) in registers, for p <= 16 with q <= 8 and
+ * p <= 32 with q <= 4 (q <= 8 in FP32). Register buckets P x Q hold the view;
+ * at p x q = P x Q the loops are guard-free, otherwise wave-uniform guards on
+ * the run-time p and q skip the rows and columns past the view (they are
+ * neither loaded, computed nor stored, so no padding enters a sum).
  *
  * The lane kernels (lane_kernels.hip) keep F in HBM and read and write it once
  * per reflector; the team kernels spread a matrix over 16 lanes and are bound
@@ -8,8 +12,8 @@
  * it in registers, stores it once, and then takes B one column at a time
  * (load, apply the q reflectors, back substitution, store), so F, tau and B
  * cross HBM once each; at V = 64 a wavefront's 64 lanes read one 512-byte run
- * per element. Every register index is a compile-time constant (p and q are
- * the template's P and Q exactly); the right-hand side count is a loop bound.
+ * per element. Every register index is a compile-time constant; the
+ * right-hand side count is a loop bound.
  *
  * The operations and their order are the lane kernel's, hence the
  * reference's, so every output is bit-identical to it:
@@ -29,6 +33,8 @@
  *          diagonal leaves B as Q^T B and reports its index + 1 (LAPACK
  *          dgels's dtrtrs test), as the lane kernel does.
  */
+
+#pragma once
 
 #include "cbk_internal.h"
 #include "team_common.h"
@@ -68,6 +74,7 @@ template <class T> struct RegQrArgs {
     T *b; /* null: geqrf */
     int b_si, b_sj;
     size_t b_gs;
+    int rows, cols; /* p x q of the view (rows >= cols) */
     int nrhs;
     bool axpy_last; /* the last column takes trsm's axpy form */
     int V;
@@ -76,10 +83,13 @@ template <class T> struct RegQrArgs {
     int *info;
 };
 
-template <class T, int P, int Q>
+template <class T, int P, int Q, bool EXACT>
 __global__ __launch_bounds__(kBlock) void qr_reg_kernel(RegQrArgs<T> p)
 {
     constexpr unsigned SZ = sizeof(T);
+    /* the view is rp x rq (rp >= rq); rows and columns past it are neither
+     * loaded, computed nor stored */
+    const int rp = EXACT ? P : p.rows, rq = EXACT ? Q : p.cols;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     /* the wavefront's first group (wave-uniform) bases the buffer resources;
      * a lane's slot is a byte offset in them, an element's place a scalar one */
@@ -97,47 +107,63 @@ __global__ __launch_bounds__(kBlock) void qr_reg_kernel(RegQrArgs<T> p)
     const __amdgpu_buffer_rsrc_t ra =
         team::group_rsrc(p.a + w0 * p.a_gs, team::group_range(groups_left, p.a_gs, SZ));
     const __amdgpu_buffer_rsrc_t rt = team::group_rsrc(
-        p.tau + w0 * (size_t)Q * p.V, team::group_range(groups_left, (size_t)Q * p.V, SZ));
+        p.tau + w0 * (size_t)rq * p.V, team::group_range(groups_left, (size_t)rq * p.V, SZ));
     const unsigned av = (unsigned)(((g - w0) * p.a_gs + v) * SZ);
-    const unsigned tv = (unsigned)(((g - w0) * (long)Q * p.V + v) * SZ);
+    const unsigned tv = (unsigned)(((g - w0) * (long)rq * p.V + v) * SZ);
     const unsigned asiV = (unsigned)p.a_si * VS, asjV = (unsigned)p.a_sj * VS;
 
     T f[P][Q];
     {
         const unsigned si = opaque(asiV), sj = opaque(asjV);
         rfor<0, P>([&](auto i) {
-            rfor<0, Q>([&](auto j) { f[i][j] = team::bload(T(), ra, av, i * si + j * sj); });
+            rfor<0, Q>([&](auto j) {
+                f[i][j] = team::bload(T(), ra, (i < rp && j < rq) ? av : team::kOOB, i * si + j * sj);
+            });
         });
     }
     /* geqrf (src/cbk_geqrf_compact.hpp:74-131) */
     T tau[Q];
     rfor<0, Q>([&](auto kc) {
         constexpr int kk = kc;
-        T tail = T(0);
-        rfor<kk + 1, P>([&](auto i) { tail = team::fmaT(f[i][kk], f[i][kk], tail); });
-        const T x0 = f[kk][kk];
-        const T norm = sqrtR(team::fmaT(x0, x0, tail));
-        const T beta = (x0 >= T(0)) ? -norm : norm;
-        const bool has = tail > T(0);
-        const T t = has ? (beta - x0) / beta : T(0);
-        const T inv = has ? T(1) / (x0 - beta) : T(0);
-        f[kk][kk] = has ? beta : x0;
-        tau[kk] = t;
-        rfor<kk + 1, P>([&](auto i) { f[i][kk] = f[i][kk] * inv; });
-        rfor<kk + 1, Q>([&](auto c) {
-            T w = f[kk][c];
-            rfor<kk + 1, P>([&](auto i) { w = team::fmaT(f[i][kk], f[i][c], w); });
-            w = w * t;
-            f[kk][c] = f[kk][c] - w;
-            rfor<kk + 1, P>([&](auto i) { f[i][c] = team::fmaT(-f[i][kk], w, f[i][c]); });
-        });
+        if (kk < rq) {
+            T tail = T(0);
+            rfor<kk + 1, P>([&](auto i) {
+                if (i < rp) tail = team::fmaT(f[i][kk], f[i][kk], tail);
+            });
+            const T x0 = f[kk][kk];
+            const T norm = sqrtR(team::fmaT(x0, x0, tail));
+            const T beta = (x0 >= T(0)) ? -norm : norm;
+            const bool has = tail > T(0);
+            const T t = has ? (beta - x0) / beta : T(0);
+            const T inv = has ? T(1) / (x0 - beta) : T(0);
+            f[kk][kk] = has ? beta : x0;
+            tau[kk] = t;
+            rfor<kk + 1, P>([&](auto i) {
+                if (i < rp) f[i][kk] = f[i][kk] * inv;
+            });
+            rfor<kk + 1, Q>([&](auto c) {
+                if (c < rq) {
+                    T w = f[kk][c];
+                    rfor<kk + 1, P>([&](auto i) {
+                        if (i < rp) w = team::fmaT(f[i][kk], f[i][c], w);
+                    });
+                    w = w * t;
+                    f[kk][c] = f[kk][c] - w;
+                    rfor<kk + 1, P>([&](auto i) {
+                        if (i < rp) f[i][c] = team::fmaT(-f[i][kk], w, f[i][c]);
+                    });
+                }
+            });
+        }
     });
     {
         const unsigned si = opaque(asiV), sj = opaque(asjV), ts = opaque(VS);
         rfor<0, P>([&](auto i) {
-            rfor<0, Q>([&](auto j) { team::bstore(f[i][j], ra, av, i * si + j * sj); });
+            rfor<0, Q>([&](auto j) {
+                team::bstore(f[i][j], ra, (i < rp && j < rq) ? av : team::kOOB, i * si + j * sj);
+            });
         });
-        rfor<0, Q>([&](auto j) { team::bstore(tau[j], rt, tv, j * ts); });
+        rfor<0, Q>([&](auto j) { team::bstore(tau[j], rt, j < rq ? tv : team::kOOB, j * ts); });
     }
     if (!p.b) return;
 
@@ -145,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void qr_reg_kernel(RegQrArgs<T> p)
     int bad = 0;
     if (p.info && idx < p.nm) {
         rfor_down<0, Q>([&](auto i) {
-            if (f[i][i] == T(0)) bad = i + 1;
+            if (i < rq && f[i][i] == T(0)) bad = i + 1;
         });
         p.info[idx] = bad;
     }
@@ -156,44 +182,59 @@ __global__ __launch_bounds__(kBlock) void qr_reg_kernel(RegQrArgs<T> p)
     for (int j = 0; j < p.nrhs; ++j) {
         const unsigned si = opaque(bsiV), col = (unsigned)j * opaque(bsjV);
         T b[P];
-        rfor<0, P>([&](auto i) { b[i] = team::bload(T(), rb, bv, col + i * si); });
+        rfor<0, P>([&](auto i) { b[i] = team::bload(T(), rb, i < rp ? bv : team::kOOB, col + i * si); });
         /* Q^T b, reflectors forward (src/cbk_ormqr_compact.hpp:68-124) */
         rfor<0, Q>([&](auto kc) {
             constexpr int kk = kc;
-            T w = b[kk];
-            rfor<kk + 1, P>([&](auto i) { w = team::fmaT(f[i][kk], b[i], w); });
-            w = w * tau[kk];
-            b[kk] = b[kk] - w;
-            rfor<kk + 1, P>([&](auto i) { b[i] = team::fmaT(-f[i][kk], w, b[i]); });
+            if (kk < rq) {
+                T w = b[kk];
+                rfor<kk + 1, P>([&](auto i) {
+                    if (i < rp) w = team::fmaT(f[i][kk], b[i], w);
+                });
+                w = w * tau[kk];
+                b[kk] = b[kk] - w;
+                rfor<kk + 1, P>([&](auto i) {
+                    if (i < rp) b[i] = team::fmaT(-f[i][kk], w, b[i]);
+                });
+            }
         });
         if (!bad) {
             if (p.axpy_last && j == p.nrhs - 1) {
                 /* trsm_axpy_col (src/cbk_trsm_compact.hpp:49-67) */
                 rfor_down<0, Q>([&](auto kc) {
                     constexpr int kk = kc;
-                    const T xk = b[kk] / f[kk][kk];
-                    b[kk] = xk;
-                    rfor<0, kk>([&](auto i) { b[i] = team::fmaT(-f[i][kk], xk, b[i]); });
+                    if (kk < rq) {
+                        const T xk = b[kk] / f[kk][kk];
+                        b[kk] = xk;
+                        rfor<0, kk>([&](auto i) { b[i] = team::fmaT(-f[i][kk], xk, b[i]); });
+                    }
                 });
             }
             else {
                 /* trsm_dot_block, one column (src/cbk_trsm_compact.hpp:73-104) */
                 rfor_down<0, Q>([&](auto ic) {
                     constexpr int i = ic;
-                    T w = b[i];
-                    rfor<i + 1, Q>([&](auto l) { w = team::fmaT(-f[i][l], b[l], w); });
-                    b[i] = w / f[i][i];
+                    if (i < rq) {
+                        T w = b[i];
+                        rfor<i + 1, Q>([&](auto l) {
+                            if (l < rq) w = team::fmaT(-f[i][l], b[l], w);
+                        });
+                        b[i] = w / f[i][i];
+                    }
                 });
             }
         }
-        rfor<0, P>([&](auto i) { team::bstore(b[i], rb, bv, col + i * si); });
+        rfor<0, P>([&](auto i) { team::bstore(b[i], rb, i < rp ? bv : team::kOOB, col + i * si); });
     }
 }
 
 template <class T, int P, int Q> hipError_t launch(const RegQrArgs<T> &a)
 {
     const unsigned nb = (unsigned)((a.nlanes + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((qr_reg_kernel<T, P, Q>), dim3(nb), dim3(kBlock), 0, current_stream(), a);
+    if (a.rows == P && a.cols == Q)
+        hipLaunchKernelGGL((qr_reg_kernel<T, P, Q, true>), dim3(nb), dim3(kBlock), 0, current_stream(), a);
+    else
+        hipLaunchKernelGGL((qr_reg_kernel<T, P, Q, false>), dim3(nb), dim3(kBlock), 0, current_stream(), a);
     return hipGetLastError();
 }
 
@@ -202,11 +243,12 @@ template <class T, int P, int Q> hipError_t launch(const RegQrArgs<T> &a)
  * registers: the columns j >= k start as e_j, then for kk = k-1 down to 0 the
  * reflector kk is applied to the columns right of it by the same larf and its
  * own column becomes -tau v below the diagonal, 1 - tau on it and 0 above */
-template <class T, int P, int Q>
-__global__ __launch_bounds__(kBlock) void orgqr_reg_kernel(int k, Operand<T> A, const T *tau_p,
-                                                           int V, long nlanes)
+template <class T, int P, int Q, bool EXACT>
+__global__ __launch_bounds__(kBlock) void orgqr_reg_kernel(int m_arg, int n_arg, int k, Operand<T> A,
+                                                           const T *tau_p, int V, long nlanes)
 {
     constexpr unsigned SZ = sizeof(T);
+    const int rp = EXACT ? P : m_arg, rq = EXACT ? Q : n_arg;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long w0 = __builtin_amdgcn_readfirstlane(
         (int)(((long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) / V));
@@ -233,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void orgqr_reg_kernel(int k, Operand<T> A, 
         rfor<0, P>([&](auto i) {
             rfor<0, Q>([&](auto j) {
                 /* only the reflector bodies (columns < k) are read */
-                f[i][j] = team::bload(T(), ra, j < k ? av : team::kOOB, i * si + j * sj);
+                f[i][j] = team::bload(T(), ra, (j < k && i < rp) ? av : team::kOOB, i * si + j * sj);
                 if (j >= k) f[i][j] = (i == j) ? T(1) : T(0);
             });
         });
@@ -243,14 +285,22 @@ __global__ __launch_bounds__(kBlock) void orgqr_reg_kernel(int k, Operand<T> A, 
         if (kk < k) {
             const T t = team::bload(T(), rt, tv, (unsigned)kk * opaque(VS));
             rfor<kk + 1, Q>([&](auto c) {
-                T w = f[kk][c];
-                rfor<kk + 1, P>([&](auto i) { w = team::fmaT(f[i][kk], f[i][c], w); });
-                w = w * t;
-                f[kk][c] = f[kk][c] - w;
-                rfor<kk + 1, P>([&](auto i) { f[i][c] = team::fmaT(-f[i][kk], w, f[i][c]); });
+                if (c < rq) {
+                    T w = f[kk][c];
+                    rfor<kk + 1, P>([&](auto i) {
+                        if (i < rp) w = team::fmaT(f[i][kk], f[i][c], w);
+                    });
+                    w = w * t;
+                    f[kk][c] = f[kk][c] - w;
+                    rfor<kk + 1, P>([&](auto i) {
+                        if (i < rp) f[i][c] = team::fmaT(-f[i][kk], w, f[i][c]);
+                    });
+                }
             });
             const T mt = -t;
-            rfor<kk + 1, P>([&](auto i) { f[i][kk] = f[i][kk] * mt; });
+            rfor<kk + 1, P>([&](auto i) {
+                if (i < rp) f[i][kk] = f[i][kk] * mt;
+            });
             f[kk][kk] = T(1) - t;
             rfor<0, kk>([&](auto i) { f[i][kk] = T(0); });
         }
@@ -258,31 +308,42 @@ __global__ __launch_bounds__(kBlock) void orgqr_reg_kernel(int k, Operand<T> A, 
     {
         const unsigned si = opaque(asiV), sj = opaque(asjV);
         rfor<0, P>([&](auto i) {
-            rfor<0, Q>([&](auto j) { team::bstore(f[i][j], ra, av, i * si + j * sj); });
+            rfor<0, Q>([&](auto j) {
+                team::bstore(f[i][j], ra, (i < rp && j < rq) ? av : team::kOOB, i * si + j * sj);
+            });
         });
     }
 }
 
 template <class T, int P, int Q>
-hipError_t launch_orgqr(int k, Operand<T> A, const T *tau, int V, long nl)
+hipError_t launch_orgqr(int m, int n, int k, Operand<T> A, const T *tau, int V, long nl)
 {
     const unsigned nb = (unsigned)((nl + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((orgqr_reg_kernel<T, P, Q>), dim3(nb), dim3(kBlock), 0, current_stream(),
-                       k, A, tau, V, nl);
+    if (m == P && n == Q)
+        hipLaunchKernelGGL((orgqr_reg_kernel<T, P, Q, true>), dim3(nb), dim3(kBlock), 0,
+                           current_stream(), m, n, k, A, tau, V, nl);
+    else
+        hipLaunchKernelGGL((orgqr_reg_kernel<T, P, Q, false>), dim3(nb), dim3(kBlock), 0,
+                           current_stream(), m, n, k, A, tau, V, nl);
     return hipGetLastError();
 }
 
-} /* namespace */
+template <int X> using ic = std::integral_constant<int, X>;
 
-/* the shapes instantiated: F's p x q fits a lane's registers without spills */
-bool qr_reg_supports(int p, int q, bool fp64)
+/* the register bucket P x Q (8x4, 8x8, 16x4, 16x8, 32x4, FP32 32x8) that
+ * holds a p x q view: its kernel runs guard-free when p x q is the bucket
+ * itself, with wave-uniform row and column guards otherwise */
+template <class T, class F> hipError_t by_bucket(int p, int q, F &&go)
 {
-    if (p == 16 && q == 8) return true;
-    if (p == 8 && (q == 4 || q == 8)) return true;
-    if (p == 16 && q == 4) return true;
-    if (p == 32 && q == 4) return true;
-    return !fp64 && p == 32 && q == 8;
+    if (p <= 8) return q <= 4 ? go(ic<8>{}, ic<4>{}) : go(ic<8>{}, ic<8>{});
+    if (p <= 16) return q <= 4 ? go(ic<16>{}, ic<4>{}) : go(ic<16>{}, ic<8>{});
+    if constexpr (sizeof(T) == 4) {
+        if (q > 4) return go(ic<32>{}, ic<8>{});
+    }
+    return go(ic<32>{}, ic<4>{});
 }
+
+} /* namespace */
 
 template <class T>
 hipError_t launch_qr_reg(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
@@ -312,39 +373,18 @@ hipError_t launch_qr_reg(int p, int q, int nrhs, bool transposed, Operand<T> A, 
     a.nlanes = (long)((nm + V - 1) / V) * V;
     a.nm = nm;
     a.info = B.p ? info : nullptr;
-    if (p == 16 && q == 8) return launch<T, 16, 8>(a);
-    if (p == 8 && q == 4) return launch<T, 8, 4>(a);
-    if (p == 8 && q == 8) return launch<T, 8, 8>(a);
-    if (p == 16 && q == 4) return launch<T, 16, 4>(a);
-    if (p == 32 && q == 4) return launch<T, 32, 4>(a);
-    if constexpr (sizeof(T) == 4) {
-        if (p == 32 && q == 8) return launch<T, 32, 8>(a);
-    }
-    return hipErrorInvalidValue;
+    a.rows = p;
+    a.cols = q;
+    if (!qr_reg_supports(p, q, sizeof(T) == 8)) return hipErrorInvalidValue;
+    return by_bucket<T>(p, q, [&](auto P, auto Q) { return launch<T, P, Q>(a); });
 }
-
-template hipError_t launch_qr_reg<double>(int, int, int, bool, Operand<double>, double *,
-                                          Operand<double>, int, int, int *);
-template hipError_t launch_qr_reg<float>(int, int, int, bool, Operand<float>, float *,
-                                         Operand<float>, int, int, int *);
 
 template <class T>
 hipError_t launch_orgqr_reg(int m, int n, int k, Operand<T> A, const T *tau, int V, int nm)
 {
     const long nl = (long)((nm + V - 1) / V) * V;
-    if (m == 16 && n == 8) return launch_orgqr<T, 16, 8>(k, A, tau, V, nl);
-    if (m == 8 && n == 4) return launch_orgqr<T, 8, 4>(k, A, tau, V, nl);
-    if (m == 8 && n == 8) return launch_orgqr<T, 8, 8>(k, A, tau, V, nl);
-    if (m == 16 && n == 4) return launch_orgqr<T, 16, 4>(k, A, tau, V, nl);
-    if (m == 32 && n == 4) return launch_orgqr<T, 32, 4>(k, A, tau, V, nl);
-    if constexpr (sizeof(T) == 4) {
-        if (m == 32 && n == 8) return launch_orgqr<T, 32, 8>(k, A, tau, V, nl);
-    }
-    return hipErrorInvalidValue;
+    if (!qr_reg_supports(m, n, sizeof(T) == 8)) return hipErrorInvalidValue;
+    return by_bucket<T>(m, n, [&](auto P, auto Q) { return launch_orgqr<T, P, Q>(m, n, k, A, tau, V, nl); });
 }
-
-template hipError_t launch_orgqr_reg<double>(int, int, int, Operand<double>, const double *, int,
-                                             int);
-template hipError_t launch_orgqr_reg<float>(int, int, int, Operand<float>, const float *, int, int);
 
 } /* namespace cbk */

@@ -1,6 +1,7 @@
 """GPU parity of the register QR kernels (qr_reg.hip: geqrf, the
 overdetermined gels and orgqr with one lane per matrix and the tall view in registers,
-p x q in {8x4, 8x8, 16x4, 16x8, 32x4}, and 32x8 in FP32) against the oracle,
+p <= 16 with q <= 8, p <= 32 with q <= 4 (q <= 8 in FP32), exact register
+buckets and guarded shapes inside them) against the oracle,
 bit for bit: it performs the lane kernel's operations in the same order, so
 every output buffer (factor, tau, solution and residual rows, the leading-
 dimension padding and the padded group slots) must equal the oracle's exactly.
@@ -32,8 +33,11 @@ def _check(case):
                                        case["V"], case["nm"])
 
 
+# exact buckets, then guarded shapes inside them
 SHAPES = [("d", 16, 8), ("s", 16, 8), ("d", 8, 4), ("s", 8, 8), ("d", 8, 8), ("d", 16, 4),
-          ("s", 16, 4), ("d", 32, 4), ("s", 32, 4), ("s", 32, 8), ("s", 8, 4)]
+          ("s", 16, 4), ("d", 32, 4), ("s", 32, 4), ("s", 32, 8), ("s", 8, 4),
+          ("d", 12, 5), ("s", 7, 3), ("d", 30, 3), ("s", 25, 7), ("d", 16, 7), ("d", 5, 5),
+          ("s", 13, 8), ("d", 3, 2)]
 
 
 @pytest.mark.parametrize("p,m,n", SHAPES)
