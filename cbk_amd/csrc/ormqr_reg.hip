@@ -1,6 +1,10 @@
 /* ormqr_reg.hip -- ormqr_compact (C := op(Q) C or C op(Q)) with one lane per
  * matrix and a block of C's columns in registers, for reflector lengths
- * nq in {8, 16, 24} (and 32 in FP32).
+ * nq <= 24 (and nq <= 32 in FP32): register buckets NQ in {8, 16, 24, 32},
+ * guard-free at nq == NQ, with wave-uniform row guards on the run-time nq
+ * below it (rows past nq are neither loaded, computed nor stored; the guarded
+ * instances run at one wavefront per SIMD fewer, which keeps them free of
+ * scratch).
  *
  * The lane kernel (lane_kernels.hip, ormqr_lane) applies each reflector in
  * place in HBM, so every element of C is read and written once per
@@ -9,8 +13,7 @@
  * tau(kk) from HBM once per column block (at V = 64 a wavefront's 64 lanes read
  * one 512-byte run per element), and stores C' once. The operations are the
  * reference's larf (src/cbk_ormqr_compact.hpp:68-105), per element in the same
- * order as ormqr_lane (nq is the compile-time NQ exactly, so no row needs a
- * guard):
+ * order as ormqr_lane:
  *     w = C'(kk,q) + sum_{i>kk} V(i,kk) C'(i,q)  (i ascending),  w *= tau(kk),
  *     C'(kk,q) -= w,  C'(i,q) -= V(i,kk) w,
  * reflectors forward when side L and trans T (or side R and N), backward
@@ -49,13 +52,14 @@ __device__ __forceinline__ float fmaR(float a, float b, float c) { return __buil
 
 constexpr int kBlock = 256;
 
-template <class T, int NQ, int NCB>
+template <class T, int NQ, int NCB, bool EXACT>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(NQ * NCB * sizeof(T) <= 256 ? 3 : 2))) void ormqr_reg_kernel(bool forward, bool ctr, int nq, int ncols,
+__attribute__((amdgpu_waves_per_eu((NQ * NCB * sizeof(T) <= 256 ? 3 : 2) - (EXACT ? 0 : 1)))) void ormqr_reg_kernel(bool forward, bool ctr, int nq_arg, int ncols,
                                                            int k, Operand<T> A, const T *tau,
                                                            Operand<T> C, int V, long nlanes)
 {
     constexpr unsigned SZ = sizeof(T);
+    const int nq = EXACT ? NQ : nq_arg;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     /* the wavefront's first group (wave-uniform) addresses a buffer resource;
      * a lane's slot is a byte offset in it, an element's place a scalar one */
@@ -98,7 +102,7 @@ __attribute__((amdgpu_waves_per_eu(NQ * NCB * sizeof(T) <= 256 ? 3 : 2))) void o
         rfor<0, NQ>([&](auto i) {
             rfor<0, NCB>([&](auto q) {
                 /* a column past ncols reads zeros (out of the resource's range) */
-                c[i][q] = team::bload(T(), rc, j0 + q < ncols ? cv : team::kOOB, i * si + (j0 + q) * sj);
+                c[i][q] = team::bload(T(), rc, (j0 + q < ncols && i < nq) ? cv : team::kOOB, i * si + (j0 + q) * sj);
             });
         });
     }
@@ -108,14 +112,18 @@ __attribute__((amdgpu_waves_per_eu(NQ * NCB * sizeof(T) <= 256 ? 3 : 2))) void o
             const T t = team::bload(T(), rt, tv, (unsigned)kk * VS);
             const unsigned si = opaque(asiV), col = kk * opaque(asjV);
             T vv[NQ];
-            rfor<kk + 1, NQ>([&](auto i) { vv[i] = team::bload(T(), ra, av, col + i * si); });
+            rfor<kk + 1, NQ>([&](auto i) { vv[i] = team::bload(T(), ra, i < nq ? av : team::kOOB, col + i * si); });
             __builtin_amdgcn_sched_barrier(0); /* one reflector's loads in flight at a time */
             rfor<0, NCB>([&](auto q) {
                 T w = c[kk][q];
-                rfor<kk + 1, NQ>([&](auto i) { w = fmaR(vv[i], c[i][q], w); });
+                rfor<kk + 1, NQ>([&](auto i) {
+                    if (i < nq) w = fmaR(vv[i], c[i][q], w);
+                });
                 w = w * t;
                 c[kk][q] = c[kk][q] - w;
-                rfor<kk + 1, NQ>([&](auto i) { c[i][q] = fmaR(-vv[i], w, c[i][q]); });
+                rfor<kk + 1, NQ>([&](auto i) {
+                    if (i < nq) c[i][q] = fmaR(-vv[i], w, c[i][q]);
+                });
             });
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -128,7 +136,7 @@ __attribute__((amdgpu_waves_per_eu(NQ * NCB * sizeof(T) <= 256 ? 3 : 2))) void o
         const unsigned si = opaque(csiV), sj = opaque(csjV);
         rfor<0, NQ>([&](auto i) {
             rfor<0, NCB>([&](auto q) {
-                team::bstore(c[i][q], rc, j0 + q < ncols ? cv : team::kOOB, i * si + (j0 + q) * sj);
+                team::bstore(c[i][q], rc, (j0 + q < ncols && i < nq) ? cv : team::kOOB, i * si + (j0 + q) * sj);
             });
         });
     }
@@ -139,15 +147,19 @@ hipError_t launch(bool forward, bool ctr, int nq, int ncols, int k, Operand<T> A
                   Operand<T> C, int V, long nl)
 {
     const dim3 grid((unsigned)((nl + kBlock - 1) / kBlock), (unsigned)((ncols + NCB - 1) / NCB));
-    hipLaunchKernelGGL((ormqr_reg_kernel<T, NQ, NCB>), grid, dim3(kBlock), 0, current_stream(),
-                       forward, ctr, nq, ncols, k, A, tau, C, V, nl);
+    if (nq == NQ)
+        hipLaunchKernelGGL((ormqr_reg_kernel<T, NQ, NCB, true>), grid, dim3(kBlock), 0,
+                           current_stream(), forward, ctr, nq, ncols, k, A, tau, C, V, nl);
+    else
+        hipLaunchKernelGGL((ormqr_reg_kernel<T, NQ, NCB, false>), grid, dim3(kBlock), 0,
+                           current_stream(), forward, ctr, nq, ncols, k, A, tau, C, V, nl);
     return hipGetLastError();
 }
 
 } /* namespace */
 
-/* FP64 at nq = 32 would spill (its 16-lane team kernel serves it) */
-bool ormqr_reg_supports(int nq, bool fp64) { return nq == 8 || nq == 16 || nq == 24 || (!fp64 && nq == 32); }
+/* FP64 past nq = 24 would spill (the 16-lane team kernel serves nq = 32) */
+bool ormqr_reg_supports(int nq, bool fp64) { return nq >= 1 && nq <= (fp64 ? 24 : 32); }
 
 template <class T>
 hipError_t launch_ormqr_reg(bool left, bool tran, int m, int n, int k, Operand<T> A,
@@ -156,15 +168,13 @@ hipError_t launch_ormqr_reg(bool left, bool tran, int m, int n, int k, Operand<T
     const long nl = (long)((nm + V - 1) / V) * V;
     const int nq = left ? m : n, ncols = left ? n : m;
     const bool forward = (left == tran);
-    switch (nq) {
-    case 8: return launch<T, 8, 4>(forward, !left, nq, ncols, k, A, tau, C, V, nl);
-    case 16: return launch<T, 16, 2>(forward, !left, nq, ncols, k, A, tau, C, V, nl);
-    case 24: return launch<T, 24, 2>(forward, !left, nq, ncols, k, A, tau, C, V, nl);
-    case 32:
-        if constexpr (sizeof(T) == 4) return launch<T, 32, 2>(forward, !left, nq, ncols, k, A, tau, C, V, nl);
-        else return hipErrorInvalidValue;
-    default: return hipErrorInvalidValue;
+    if (nq <= 8) return launch<T, 8, 4>(forward, !left, nq, ncols, k, A, tau, C, V, nl);
+    if (nq <= 16) return launch<T, 16, 2>(forward, !left, nq, ncols, k, A, tau, C, V, nl);
+    if (nq <= 24) return launch<T, 24, 2>(forward, !left, nq, ncols, k, A, tau, C, V, nl);
+    if constexpr (sizeof(T) == 4) {
+        if (nq <= 32) return launch<T, 32, 2>(forward, !left, nq, ncols, k, A, tau, C, V, nl);
     }
+    return hipErrorInvalidValue;
 }
 
 template hipError_t launch_ormqr_reg<double>(bool, bool, int, int, int, Operand<double>,
