@@ -539,7 +539,7 @@ def main():
     Xp = torch.zeros_like(Bp)
     iters = torch.zeros(nm, dtype=torch.int32, device="cuda")
     info = torch.zeros(nm, dtype=torch.int32, device="cuda")
-    ws = cbk_amd.dsposv_worksize("C", "L", N, NRHS, V, nm)
+    ws = cbk_amd.dsposv_worksize("C", "L", N, NRHS, N, N, N, V, nm)
     work = torch.empty(max(ws, 8), dtype=torch.uint8, device="cuda") if ws else None
     Am = Ap.clone()
 

@@ -105,12 +105,11 @@ def call_ext(name, *args):
     return f(*[_ptr(a) if k == "p" else abi.to_c(k, a) for k, a in zip(kinds, args)])
 
 
-def dsposv_worksize(layout, uplo, n, nrhs, V, nm):
+def dsposv_worksize(layout, uplo, n, nrhs, ldap, ldbp, ldxp, V, nm):
     f = lib().dsposv_compact_worksize
-    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                  ctypes.c_int]
+    f.argtypes = [ctypes.c_char, ctypes.c_char] + [ctypes.c_int] * 7
     f.restype = ctypes.c_size_t
-    return f(abi.to_c("c", layout), abi.to_c("c", uplo), n, nrhs, V, nm)
+    return f(abi.to_c("c", layout), abi.to_c("c", uplo), n, nrhs, ldap, ldbp, ldxp, V, nm)
 
 
 def set_stream(stream):

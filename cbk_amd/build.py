@@ -32,21 +32,36 @@ CFLAGS = [
 ]
 
 
-def _compile(src):
+def _up_to_date(src, obj):
+    """obj is newer than src and every header its dependency file lists"""
+    dep = obj[:-2] + ".d"
+    if not (os.path.exists(obj) and os.path.exists(dep)):
+        return False
+    text = open(dep).read().replace("\\\n", " ")
+    files = text.split(":", 1)[1].split() if ":" in text else [src]
+    t = os.path.getmtime(obj)
+    return all(os.path.exists(f) and os.path.getmtime(f) < t for f in files + [src])
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    if incremental and _up_to_date(src, obj):
+        return obj
     lang = ["-x", "hip"] if src.endswith((".hip", ".cpp")) else []
-    cmd = [HIPCC] + CFLAGS + lang + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + CFLAGS + lang + ["-MMD", "-MF", obj[:-2] + ".d", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def build(verbose=True):
+def build(verbose=True, incremental=False):
+    """Compile every source (incremental: only those older than their object
+    or one of its headers) and link the library."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}",
            "-Wl,--version-script=" + os.path.join(CSRC, "exports.map"), "-o", tmp] + objs
@@ -60,5 +75,5 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)

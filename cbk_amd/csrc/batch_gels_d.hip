@@ -37,7 +37,8 @@ hipError_t launch_qr_wave_masked(int p, int q, int nrhs, Operand<double> A, Oper
     args.x_rs = (X.row ? X.ld : 1) * V;
     args.x_cs = (X.row ? 1 : X.ld) * V;
     args.x_gs = (long long)X.gstride;
-    const unsigned blocks = team::xcd_grid((unsigned)((nm + 3) / 4));
+    /* persistent: one wavefront per 64-entry chunk of the mask */
+    const unsigned blocks = (unsigned)(((nm + 63) / 64 + 3) / 4);
     if (p <= 32)
         hipLaunchKernelGGL((batch::gels_wave_kernel<double, 32>), dim3(blocks), dim3(256), 0,
                            current_stream(), args);

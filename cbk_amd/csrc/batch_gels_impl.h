@@ -208,11 +208,31 @@ gels_wave_kernel(BatchArgs<T> p)
     __shared__ T vshare[4][MMAX];
     T *vb = vshare[wave_in_block()]; /* this wavefront's broadcast row */
     const int c = threadIdx.x & 63;
+    /* masked compact mode: the grid is persistent and wavefront w scans the
+     * mask 64 entries at a time (chunks w, w + waves, ...), one coalesced load
+     * and a ballot per chunk, then solves the marked matrices of the chunk
+     * one after another; a call with no marked matrix costs one pass over
+     * the mask */
+    const unsigned nwaves = gridDim.x * 4u;
+    unsigned chunk = blockIdx.x * 4u + (unsigned)wave_in_block();
+    unsigned long long pend = 0;
+    bool first_chunk = true;
     for (int round = 0;; ++round) {
         unsigned kq = 0;
         if (p.counter) {
             if (c == 0) kq = atomicAdd(p.counter, 1u);
             kq = __builtin_amdgcn_readfirstlane(kq);
+        }
+        else if (p.mask) {
+            while (pend == 0) {
+                if (!first_chunk) chunk += nwaves;
+                first_chunk = false;
+                if (chunk >= ((unsigned)p.ninfo + 63u) / 64u) return;
+                const unsigned k = chunk * 64u + (unsigned)c;
+                pend = __builtin_amdgcn_ballot_w64(k < (unsigned)p.ninfo && p.mask[k] < 0);
+            }
+            kq = chunk * 64u + (unsigned)__builtin_ctzll(pend);
+            pend &= pend - 1ull;
         }
         else {
             /* compact mode: one problem per wavefront; blocks in XCD order, so
@@ -222,7 +242,6 @@ gels_wave_kernel(BatchArgs<T> p)
             kq = xcd_block(blockIdx.x) * 4u + (unsigned)wave_in_block();
         }
         if (kq >= (unsigned)p.count) return;
-        if (p.mask && (kq >= (unsigned)p.ninfo || p.mask[kq] >= 0)) continue;
         const BatchProblem d = problem_at(p, kq);
         const int m = d.m, n = d.n, w = d.n + d.nrhs;
         const bool isA = c < n, isB = c >= n && c < w;

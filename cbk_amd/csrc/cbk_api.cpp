@@ -92,9 +92,10 @@ int ormqr(char layout, char side, char trans, int m, int n, int k, const T *ap, 
      * (src/cbk_ormqr_compact.hpp:144) */
     const Operand<T> A = operand(const_cast<T *>(ap), ldap, row, nq, k, V);
     const Operand<T> C = operand(cp, ldcp, row, m, n, V);
-    if (current_policy() != CBK_POLICY_LANE && ormqr_reg_supports(nq, sizeof(T) == 8))
+    const bool fit = offsets_fit(A, nq, k, V) && offsets_fit(C, m, n, V);
+    if (fit && current_policy() != CBK_POLICY_LANE && ormqr_reg_supports(nq, sizeof(T) == 8))
         return status(launch_ormqr_reg<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
-    if (current_policy() != CBK_POLICY_LANE && V >= 4 && ormqr_team_supports(nq))
+    if (fit && current_policy() != CBK_POLICY_LANE && V >= 4 && ormqr_team_supports(nq))
         return status(launch_ormqr_team<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
     return status(launch_ormqr_lane<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
 }
@@ -113,7 +114,8 @@ int orgqr(char layout, int m, int n, int k, T *ap, int ldap, const T *taup, int 
     if (nm < 0) return -9;
     if (m == 0 || n == 0 || nm == 0) return 0; /* k == 0 is not empty */
     const Operand<T> A = operand(ap, ldap, row, m, n, V);
-    if (current_policy() != CBK_POLICY_LANE && qr_reg_supports(m, n, sizeof(T) == 8))
+    if (offsets_fit(A, m, n, V) && current_policy() != CBK_POLICY_LANE &&
+        qr_reg_supports(m, n, sizeof(T) == 8))
         return status(launch_orgqr_reg<T>(m, n, k, A, taup, V, nm));
     return status(launch_orgqr_lane<T>(m, n, k, A, taup, V, nm));
 }
@@ -259,14 +261,19 @@ int gels(char layout, char trans, int m, int n, int nrhs, T *ap, int ldap, T *bp
 
 constexpr int kItermax = 30; /* LAPACK dsposv's ITERMAX */
 
-bool mixed_uses_team(int n, int nrhs, int V)
+/* the fused kernel serves the call: its shapes, V >= 4, and operands whose
+ * 32-bit offsets fit (offsets_fit) */
+bool mixed_uses_team(bool row, int n, int nrhs, int ldap, int ldbp, int ldxp, int V)
 {
-    return current_policy() == CBK_POLICY_AUTO && V >= 4 && mixed_team_supports(n, nrhs);
+    return current_policy() == CBK_POLICY_AUTO && V >= 4 && mixed_team_supports(n, nrhs) &&
+           offsets_fit_bytes(gstride(row, ldap, n, n, V), ldap, row, n, n, V, 8) &&
+           offsets_fit_bytes(gstride(row, ldbp, n, nrhs, V), ldbp, row, n, nrhs, V, 8) &&
+           offsets_fit_bytes(gstride(row, ldxp, n, nrhs, V), ldxp, row, n, nrhs, V, 8);
 }
 
-size_t mixed_worksize(int n, int nrhs, int V, int nm)
+size_t mixed_worksize(bool row, int n, int nrhs, int ldap, int ldbp, int ldxp, int V, int nm)
 {
-    if (mixed_uses_team(n, nrhs, V)) return 0;
+    if (mixed_uses_team(row, n, nrhs, ldap, ldbp, ldxp, V)) return 0;
     const size_t groups = (size_t)((nm + V - 1) / V);
     return groups * V * ((size_t)n * n + (size_t)n * nrhs) * sizeof(float);
 }
@@ -284,7 +291,10 @@ int dsposv(char layout, char uplo, int n, int nrhs, double *ap, int ldap, const 
     if (ldap < max1(n)) return -6;
     if (ldbp < max1(row ? nrhs : n)) return -8;
     if (ldxp < max1(row ? nrhs : n)) return -10;
-    if (vlen_ok(V) && nm >= 0 && lwork < mixed_worksize(n, nrhs, V, nm)) return -12;
+    if (vlen_ok(V) && nm >= 0 && lwork < mixed_worksize(row, n, nrhs, ldap, ldbp, ldxp, V, nm))
+        return -12;
+    if (nm > 0 && iter == nullptr) return -13;
+    if (nm > 0 && info == nullptr) return -14;
     if (!vlen_ok(V)) return -15;
     if (nm < 0) return -16;
     if (n == 0 || nrhs == 0 || nm == 0) {
@@ -295,7 +305,7 @@ int dsposv(char layout, char uplo, int n, int nrhs, double *ap, int ldap, const 
     const Operand<double> A = operand(ap, ldap, row, n, n, V);
     const Operand<double> B = operand(const_cast<double *>(bp), ldbp, row, n, nrhs, V);
     const Operand<double> X = operand(xp, ldxp, row, n, nrhs, V);
-    if (mixed_uses_team(n, nrhs, V))
+    if (mixed_uses_team(row, n, nrhs, ldap, ldbp, ldxp, V))
         return status(launch_mixed_posv_team(up, n, nrhs, A, B, X, V, nm, kItermax, iter, info));
     return status(launch_mixed_posv_lane(up, n, nrhs, A, B, X, static_cast<float *>(work), V,
                                          nm, kItermax, iter, info));
@@ -316,6 +326,7 @@ int dsgels(char layout, char trans, int m, int n, int nrhs, double *ap, int ldap
     if (ldap < max1(row ? n : m)) return -7;
     if (ldbp < max1(row ? nrhs : m)) return -9;
     if (ldxp < max1(row ? nrhs : n)) return -11;
+    if (nm > 0 && iter == nullptr) return -12;
     if (!vlen_ok(V)) return -14;
     if (nm < 0) return -15;
     hipError_t e = launch_clear_info(info, nm);
@@ -327,7 +338,8 @@ int dsgels(char layout, char trans, int m, int n, int nrhs, double *ap, int ldap
     const Operand<double> B = operand(const_cast<double *>(bp), ldbp, row, m, nrhs, V);
     const Operand<double> X = operand(xp, ldxp, row, n, nrhs, V);
     if (e == hipSuccess) {
-        if (current_policy() != CBK_POLICY_LANE && V >= 4 && mixed_gels_supports(m, n, nrhs))
+        if (current_policy() != CBK_POLICY_LANE && V >= 4 && mixed_gels_supports(m, n, nrhs) &&
+            offsets_fit(A, m, n, V) && offsets_fit(B, m, nrhs, V) && offsets_fit(X, n, nrhs, V))
             e = launch_mixed_gels_team(m, n, nrhs, A, B, X, V, nm, kItermax, iter);
         else
             e = launch_fill_int(iter, nm, -1);
@@ -468,12 +480,12 @@ int dsgels_compact(char layout, char trans, int m, int n, int nrhs, double *ap, 
     return dsgels(layout, trans, m, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, iter, info, V, nm);
 }
 
-size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int V, int nm)
+size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int ldap, int ldbp,
+                               int ldxp, int V, int nm)
 {
-    (void)layout;
     (void)uplo;
     if (n < 0 || nrhs < 0 || nm < 0 || !vlen_ok(V)) return 0;
-    return mixed_worksize(n, nrhs, V, nm);
+    return mixed_worksize(opt(layout, 'R'), n, nrhs, ldap, ldbp, ldxp, V, nm);
 }
 
 CBK_ENTRY_POINTS(double, d)

@@ -7,6 +7,8 @@
 
 #include <cstddef>
 
+#include "offsets.h"
+
 namespace cbk {
 
 /* the stream of the calling host thread (cbk_set_stream) */
@@ -27,6 +29,11 @@ template <class T> struct Operand {
     bool row;
     size_t gstride;
 };
+
+template <class T> inline bool offsets_fit(const Operand<T> &o, int rows, int cols, int V)
+{
+    return o.p == nullptr || offsets_fit_bytes(o.gstride, o.ld, o.row, rows, cols, V, sizeof(T));
+}
 
 /* ---- generic lane kernels: one lane = one matrix, any V, any layout ---- */
 

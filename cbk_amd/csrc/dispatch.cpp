@@ -10,11 +10,13 @@ template <class T>
 hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
                         Operand<T> B, int V, int nm, int *info)
 {
+    /* 32-bit offsets of the register and team kernels (offsets_fit) */
+    const bool fit = offsets_fit(A, n, n, V) && offsets_fit(B, n, nrhs, V);
     /* the whole triangle in one lane's registers where it fits (measured
      * faster than the team kernel at its orders 16 and 24 too) */
-    if (current_policy() != CBK_POLICY_LANE && reg_chol_supports(n, sizeof(T) == 8))
+    if (fit && current_policy() != CBK_POLICY_LANE && reg_chol_supports(n, sizeof(T) == 8))
         return launch_reg_chol<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
-    if (current_policy() == CBK_POLICY_AUTO && sym_team_supports(n) && V >= 4)
+    if (fit && current_policy() == CBK_POLICY_AUTO && sym_team_supports(n) && V >= 4)
         return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
 }
@@ -23,9 +25,11 @@ template <class T>
 hipError_t dispatch_trsm(bool left, bool up, bool tran, bool unit, int m, int n, T alpha,
                          Operand<T> A, Operand<T> B, int V, int nm)
 {
-    if (current_policy() != CBK_POLICY_LANE && trsm_reg_supports(left, m, sizeof(T) == 8))
+    const int s = left ? m : n;
+    const bool fit = offsets_fit(A, s, s, V) && offsets_fit(B, m, n, V);
+    if (fit && current_policy() != CBK_POLICY_LANE && trsm_reg_supports(left, m, sizeof(T) == 8))
         return launch_trsm_reg<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
-    if (current_policy() != CBK_POLICY_LANE && V >= 4 && trsm_team_supports(left, up, tran, m))
+    if (fit && current_policy() != CBK_POLICY_LANE && V >= 4 && trsm_team_supports(left, up, tran, m))
         return launch_trsm_team<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
     return launch_trsm_lane<T>(left, up, tran, unit, m, n, alpha, A, B, V, nm);
 }
@@ -36,7 +40,8 @@ hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operan
 {
     const bool tall = m >= n;
     const int p = tall ? m : n, q = tall ? n : m;
-    const bool overdet = (tall != tran); /* src/cbk_gels_compact.hpp:124 */
+    const bool overdet = (tall != tran) && offsets_fit(A, m, n, V) &&
+                         offsets_fit(B, p, nrhs, V); /* src/cbk_gels_compact.hpp:124 */
     /* F in one lane's registers where it fits; then the row team at the
      * orders it is tuned for (its butterfly sums measured faster there), the
      * column team for the other small shapes */
@@ -55,12 +60,13 @@ hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operan
 template <class T> hipError_t dispatch_geqrf(int m, int n, Operand<T> A, T *tau, int V, int nm)
 {
     const Operand<T> none = {nullptr, 1, A.row, 0};
-    if (current_policy() != CBK_POLICY_LANE && m >= n && qr_reg_supports(m, n, sizeof(T) == 8))
+    const bool tall = m >= n && offsets_fit(A, m, n, V);
+    if (current_policy() != CBK_POLICY_LANE && tall && qr_reg_supports(m, n, sizeof(T) == 8))
         return launch_qr_reg<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
-    if (current_policy() == CBK_POLICY_AUTO && m >= n && V >= 4 &&
+    if (current_policy() == CBK_POLICY_AUTO && tall && V >= 4 &&
         qr_team_supports(m, n, 0, sizeof(T) == 8))
         return launch_qr_team<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
-    if (current_policy() == CBK_POLICY_AUTO && m >= n && V >= 4 && qr_cols_supports(m, n, 0))
+    if (current_policy() == CBK_POLICY_AUTO && tall && V >= 4 && qr_cols_supports(m, n, 0))
         return launch_qr_cols<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
     return launch_geqrf_lane<T>(m, n, A, tau, V, nm);
 }

@@ -80,20 +80,25 @@ int sgels_compact_info(char layout, char trans, int m, int n, int nrhs, float *a
  *     iter  >= 0 refinement steps taken (A untouched);
  *           -2 overflow, -3 FP32 factor failed, -31 no convergence: the FP64
  *           path ran and A holds its Cholesky factor;
- *     info  0, or the FP64 dpotrf code of a matrix that is not SPD (X then
- *           unwritten).
+ *     info  0, or the FP64 dpotrf code of a matrix that is not SPD (X of
+ *           that matrix then unspecified: as LAPACK dsposv, it may hold an
+ *           unfinished iterate).
  * B is read-only; X (n x nrhs, ldxp, same layout) receives the solution.
+ * iter and info are required when nm > 0.
  * work: device scratch of dsposv_compact_worksize(...) bytes (0 -- and work
- * may be NULL -- when the fused kernel serves the shape: n in {16, 24, 32},
- * nrhs <= 4, V >= 4; see that function).
+ * may be NULL -- when a fused kernel serves the call; see that function).
  * Codes: -1 layout -2 uplo -3 n -4 nrhs -6 ldap -8 ldbp -10 ldxp -12 lwork
- *        -15 V -16 nm */
+ *        -13 iter NULL -14 info NULL -15 V -16 nm */
 int dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
                    const double *bp, int ldbp, double *xp, int ldxp, void *work,
                    size_t lwork, int *iter, int *info, int V, int nm);
 
-/* bytes of workspace dsposv_compact needs for this call shape (0 if none) */
-size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int V, int nm);
+/* bytes of workspace dsposv_compact needs for this call (0 if none): 0 when
+ * a fused kernel serves it (n in {16, 24, 32}, nrhs <= 4, V >= 4, and leading
+ * dimensions small enough for the kernels' 32-bit offsets -- below about
+ * 2^31 / (64 * 8 * n) at V = 64), else groups * V * (n*n + n*nrhs) * 4 */
+size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int ldap, int ldbp,
+                               int ldxp, int V, int nm);
 
 /* dsgels_compact: mixed-precision least squares min ||B - A X|| for every
  * matrix (trans 'N', n <= m <= 64, n + nrhs <= 64, full column rank). No LAPACK
@@ -113,9 +118,11 @@ size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int V, i
  *     info  0, or (FP64 path) the 1-based index of the first exact zero of
  *           R's diagonal, X then holding (Q^T B)(0:n).
  * B (m x nrhs, ldbp) is read only; X (n x nrhs, ldxp, same layout) receives
- * the solution. iter is required (device int[nm]); info may be NULL.
- * Codes: -1 layout -2 trans -3 m -4 n -5 nrhs -7 ldap -9 ldbp -11 ldxp -14 V
- *        -15 nm */
+ * the solution. iter is required (device int[nm]) when nm > 0; info may be
+ * NULL. Matrices that refine never reach the FP64 pass, whose launch then
+ * only scans iter.
+ * Codes: -1 layout -2 trans -3 m -4 n -5 nrhs -7 ldap -9 ldbp -11 ldxp
+ *        -12 iter NULL -14 V -15 nm */
 int dsgels_compact(char layout, char trans, int m, int n, int nrhs, double *ap, int ldap,
                    const double *bp, int ldbp, double *xp, int ldxp, int *iter, int *info,
                    int V, int nm);

@@ -102,3 +102,25 @@ def test_quick_returns_launch_nothing():
     assert cbk_amd.call("s", "geqrf", "C", 0, 3, None, 1, None, 64, 10) == 0
     assert cbk_amd.call("s", "ormqr", "C", "L", "T", 4, 3, 0, None, 4, None, None, 4, 64, 10) == 0
     assert cbk_amd.call("d", "trsm", "C", "L", "L", "N", "U", 0, 3, 1.0, None, 1, None, 1, 64, 5) == 0
+
+
+def test_mixed_entry_points_reject_null_outputs():
+    """dsposv_compact needs iter and info, dsgels_compact iter, when nm > 0:
+    a NULL is an argument error (returned before any launch), not a fault"""
+    z, fake = None, 8  # fake: a non-null address that is never dereferenced
+    ds = lambda *a: cbk_amd.call_ext("dsposv_compact", *a)
+    # (layout, uplo, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, work, lwork, iter, info, V, nm)
+    assert ds("C", "L", 32, 4, z, 32, z, 32, z, 32, z, 0, z, fake, 64, 10) == -13
+    assert ds("C", "L", 32, 4, z, 32, z, 32, z, 32, z, 0, fake, z, 64, 10) == -14
+    assert ds("C", "L", 32, 4, z, 32, z, 32, z, 32, z, 0, z, z, 3, 10) == -13
+    assert ds("C", "L", 32, 4, z, 32, z, 32, z, 32, z, 0, fake, fake, 3, 10) == -15
+    assert ds("C", "L", 32, 4, z, 32, z, 32, z, 32, z, 0, z, z, 64, 0) == 0  # nothing to do
+    # an ldap too large for the fused kernel's 32-bit offsets needs the lane path's workspace
+    ws = cbk_amd.dsposv_worksize("C", "L", 32, 4, 1 << 20, 32, 32, 64, 10)
+    assert ws == 64 * (32 * 32 + 32 * 4) * 4
+    assert cbk_amd.dsposv_worksize("C", "L", 32, 4, 32, 32, 32, 64, 10) == 0
+    assert ds("C", "L", 32, 4, z, 1 << 20, z, 32, z, 32, z, 0, fake, fake, 64, 10) == -12
+    dg = lambda *a: cbk_amd.call_ext("dsgels_compact", *a)
+    # (layout, trans, m, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, iter, info, V, nm)
+    assert dg("C", "N", 64, 32, 2, z, 64, z, 64, z, 32, z, z, 64, 10) == -12
+    assert dg("C", "N", 64, 32, 2, z, 64, z, 64, z, 32, z, z, 64, 0) == 0

@@ -47,7 +47,7 @@ ROUTINES = ["potrf", "sytrfnp", "posv", "potrs", "sysvnp", "sytrsnp", "trsm", "g
 
 
 @pytest.mark.parametrize("routine", ROUTINES)
-@pytest.mark.parametrize("V,nm", [(2, 5), (8, 11), (64, 130)])
+@pytest.mark.parametrize("V,nm", [(2, 5), (8, 11), (32, 71), (64, 130)])
 def test_lane_parity(routine, V, nm):
     n = 0
     for i, (rt, p, opts, dims) in enumerate(cases.sweep(quick=(V == 64))):

@@ -49,7 +49,7 @@ def run_dsposv(p_call, A, B, V, uplo="L", layout="C"):
     import cbk_amd
     import gpuutil
     torch = gpuutil.torch()
-    ws = cbk_amd.dsposv_worksize(layout, uplo, n, nrhs, V, nm)
+    ws = cbk_amd.dsposv_worksize(layout, uplo, n, nrhs, n, ldb, ldb, V, nm)
     dA, dB, dX = (gpuutil.to_dev(x) for x in (ap, bp, xp))
     dI, dN = gpuutil.to_dev(iters), gpuutil.to_dev(info)
     dW = torch.empty(max(ws, 8), dtype=torch.uint8, device="cuda") if ws else None
@@ -76,6 +76,8 @@ def cond1(A):
     (32, 4, 64, "L", "C", True),    # the same shape through the workspace lane kernel
     (20, 5, 8, "L", "C", False),    # an order only the lane kernel serves
     (7, 2, 2, "U", "R", False),
+    (32, 4, 32, "L", "C", False),   # V = 32: the fused kernel's matrices span two per group line
+    (24, 2, 32, "U", "R", False),
 ])
 def test_dsposv_refines_to_fp64_accuracy(n, nrhs, V, uplo, layout, lane):
     import cbk_amd

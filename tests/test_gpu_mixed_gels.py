@@ -54,7 +54,7 @@ def _run_both(layout, m, n, nrhs, V, nm, A, B):
 
 @pytest.mark.parametrize("m,n,nrhs", [(64, 32, 2), (64, 32, 1), (64, 32, 4), (32, 16, 3),
                                       (16, 8, 2), (24, 10, 2)])
-@pytest.mark.parametrize("layout,V,nm", [("C", 64, 150), ("R", 8, 21), ("C", 2, 7)])
+@pytest.mark.parametrize("layout,V,nm", [("C", 64, 150), ("R", 8, 21), ("C", 2, 7), ("C", 32, 71)])
 def test_dsgels_matches_oracle(m, n, nrhs, layout, V, nm):
     print(f"case dsgels {m}x{n} nrhs={nrhs} {layout} V={V} nm={nm}", flush=True)
     r = cbkt.rng(7000 + m + nrhs + V)
@@ -110,4 +110,5 @@ def test_dsgels_info_and_codes():
     assert cbk_amd.call_ext("dsgels_compact", "C", "N", m, m + 1, 1, z, m, z, m, z, m, z, z, V, nm) == -4
     assert cbk_amd.call_ext("dsgels_compact", "C", "N", 64, 60, 5, z, 64, z, 64, z, 60, z, z, V, nm) == -5
     assert cbk_amd.call_ext("dsgels_compact", "C", "N", m, n, 1, z, m, z, m, z, n - 1, z, z, V, nm) == -11
-    assert cbk_amd.call_ext("dsgels_compact", "C", "N", m, n, 1, z, m, z, m, z, n, z, z, 3, nm) == -14
+    assert cbk_amd.call_ext("dsgels_compact", "C", "N", m, n, 1, z, m, z, m, z, n, z, z, V, nm) == -12
+    assert cbk_amd.call_ext("dsgels_compact", "C", "N", m, n, 1, z, m, z, m, z, n, 8, z, 3, nm) == -14

@@ -32,7 +32,7 @@ def _bits(x):
 def test_ormqr_reg(p, side, m, n, k):
     for lay in ("C", "R"):
         for tr in ("T", "N"):
-            for V, nm in ((64, 130), (8, 13), (2, 5)):
+            for V, nm in ((64, 130), (32, 71), (8, 13), (2, 5)):
                 print(f"case ormqr{p} {side}{tr} {lay} m={m} n={n} k={k} V={V} nm={nm}", flush=True)
                 c = cases.make_case("ormqr", p, {"layout": lay, "side": side, "trans": tr},
                                     {"m": m, "n": n, "k": k}, V, nm, seed=7700 + m + n + k, ld_extra=1)

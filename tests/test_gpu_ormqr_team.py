@@ -24,7 +24,7 @@ def _gpu():
 def test_ormqr_team(p, side, m, n, k):
     for lay in ("C", "R"):
         for tr in ("T", "N"):
-            for V, nm in ((64, 130), (8, 13), (4, 6)):
+            for V, nm in ((64, 130), (32, 71), (8, 13), (4, 6)):
                 print(f"case ormqr{p} {side}{tr} {lay} m={m} n={n} k={k} V={V} nm={nm}", flush=True)
                 c = cases.make_case("ormqr", p, {"layout": lay, "side": side, "trans": tr},
                                     {"m": m, "n": n, "k": k}, V, nm, seed=9900 + m + n + k, ld_extra=1)

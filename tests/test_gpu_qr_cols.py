@@ -51,7 +51,7 @@ def _ls_check(p, Ad, B0, Bo, Bg, n, m, nm):
 
 
 @pytest.mark.parametrize("p,m,n,nrhs", GELS)
-@pytest.mark.parametrize("layout,trans,V", [("C", "N", 64), ("R", "N", 4), ("C", "T", 8)])
+@pytest.mark.parametrize("layout,trans,V", [("C", "N", 64), ("R", "N", 4), ("C", "T", 8), ("C", "N", 32)])
 def test_qr_cols_gels(p, m, n, nrhs, layout, trans, V):
     nm = 3 * V + 5 if V < 64 else 140
     am, an = (m, n) if trans == "N" else (n, m)

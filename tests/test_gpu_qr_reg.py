@@ -45,7 +45,7 @@ def test_qr_reg_gels(p, m, n):
     for i, (layout, trans, nrhs, V, nm) in enumerate([("C", "N", 2, 64, 130), ("C", "N", 3, 8, 13),
                                                       ("R", "N", 1, 4, 9), ("C", "T", 5, 2, 5),
                                                       ("R", "T", 2, 64, 70), ("C", "N", 1, 2, 3),
-                                                      ("R", "T", 1, 8, 11)]):
+                                                      ("R", "T", 1, 8, 11), ("C", "N", 2, 32, 71)]):
         am, an = (m, n) if trans == "N" else (n, m)
         print(f"case gels{p} {am}x{an} nrhs={nrhs} {layout}{trans} V={V} nm={nm}", flush=True)
         _check(cases.make_case("gels", p, {"layout": layout, "trans": trans},
@@ -55,7 +55,7 @@ def test_qr_reg_gels(p, m, n):
 
 @pytest.mark.parametrize("p,m,n", SHAPES)
 def test_qr_reg_geqrf(p, m, n):
-    for i, (layout, V, nm) in enumerate([("C", 64, 130), ("R", 8, 11), ("C", 2, 3)]):
+    for i, (layout, V, nm) in enumerate([("C", 64, 130), ("R", 8, 11), ("C", 2, 3), ("R", 32, 71)]):
         print(f"case geqrf{p} {m}x{n} {layout} V={V} nm={nm}", flush=True)
         _check(cases.make_case("geqrf", p, {"layout": layout}, {"m": m, "n": n}, V, nm,
                                seed=8300 + 31 * m + n + i, ld_extra=1 - i % 2))
@@ -86,7 +86,7 @@ def test_qr_reg_gels_info():
 def test_qr_reg_orgqr(p, m, n):
     """orgqr of the same shapes: all k from 0 to n over the cases, bit for bit"""
     for i, (layout, k, V, nm) in enumerate([("C", n, 64, 130), ("R", n - 1, 8, 11), ("C", n // 2, 2, 3),
-                                            ("R", 0, 4, 9), ("C", 1, 16, 17)]):
+                                            ("R", 0, 4, 9), ("C", 1, 16, 17), ("C", n, 32, 71)]):
         print(f"case orgqr{p} {m}x{n} k={k} {layout} V={V} nm={nm}", flush=True)
         _check(cases.make_case("orgqr", p, {"layout": layout}, {"m": m, "n": n, "k": k}, V, nm,
                                seed=8500 + 31 * m + n + i, ld_extra=i % 2))

@@ -26,9 +26,10 @@ def _gpu():
 
 
 @pytest.mark.parametrize("p,m,n", SHAPES)
-@pytest.mark.parametrize("layout,trans", [("C", "N"), ("R", "N"), ("C", "T")])
-def test_qr_team_gels(p, m, n, layout, trans):
-    V, nm = 64, 140
+@pytest.mark.parametrize("layout,trans,V", [("C", "N", 64), ("R", "N", 64), ("C", "T", 64),
+                                           ("C", "N", 32)])
+def test_qr_team_gels(p, m, n, layout, trans, V):
+    nm = 140 if V == 64 else 3 * V + 7
     # trans: op(A) = A^T with A n x m (m < n as stored) is the same overdetermined problem
     am, an = (m, n) if trans == "N" else (n, m)
     for nrhs in (1, 2):

@@ -30,7 +30,7 @@ def _gpu():
 
 @pytest.mark.parametrize("p,m,n,nrhs", SHAPES)
 @pytest.mark.parametrize("layout,trans,V,nm", [("C", "N", 64, 130), ("R", "N", 16, 37),
-                                               ("C", "T", 2, 9)])
+                                               ("C", "T", 2, 9), ("C", "N", 32, 71)])
 def test_qr_wave_gels(p, m, n, nrhs, layout, trans, V, nm):
     am, an = (m, n) if trans == "N" else (n, m)
     print(f"case gels-wave {p} {am}x{an} nrhs={nrhs} {layout}{trans} V={V} nm={nm}", flush=True)

@@ -24,7 +24,7 @@ def _gpu():
                          [("s", 20), ("s", 24)])
 def test_reg_factor_and_solve_bitwise(p, n):
     for i, (lay, uplo) in enumerate((("C", "L"), ("C", "U"), ("R", "L"), ("R", "U"))):
-        for V, nm in ((64, 130), (8, 13), (2, 5)):
+        for V, nm in ((64, 130), (32, 71), (8, 13), (2, 5)):
             for rt in ("potrf", "sytrfnp", "posv", "sysvnp", "potrs", "sytrsnp"):
                 for nrhs in ((0,) if rt in ("potrf", "sytrfnp") else (1, 3, 4, 5)):
                     dims = {"n": n} if nrhs == 0 else {"n": n, "nrhs": nrhs}
@@ -38,17 +38,34 @@ def test_reg_factor_and_solve_bitwise(p, n):
                         assert cases.same_bits(a, b), (rt, lay, uplo, V, nrhs)
 
 
-def test_reg_info():
-    V, nm, n = 64, 70, 8
-    c = cases.make_case("posv", "d", {"layout": "C", "uplo": "L"}, {"n": n, "nrhs": 2}, V, nm, seed=4)
-    A = cbkt.unpack(c["args"][4], nm, n, n, V)
-    A[7, 5, 5] = -3.0
-    c["args"][4] = cbkt.pack(A, V)
-    info_o = np.zeros(nm, np.int32)
-    info_g = np.zeros(nm, np.int32)
-    so, outo = cases.run(cbkt.oracle().call, c, info=info_o)
-    sg, outg = cases.run(_gpu().gpu_call, c, info=info_g)
-    assert so == sg == 0
-    assert np.array_equal(info_o, info_g) and info_o[7] == 6
-    for a, b in zip(outo, outg):
-        assert cases.same_bits_nan(a, b)
+@pytest.mark.parametrize("p,n", [("d", 8), ("d", 16), ("s", 24), ("d", 13)])
+def test_reg_info(p, n):
+    """info = k + 1 for a failed pivot k (Cholesky: a negative diagonal; LDL^T:
+    row and column k zeroed up to the diagonal, an exact zero D(k)), every k
+    of the order over the batch; failed matrices keep B"""
+    V, nm = 64, 70
+    for rt in ("posv", "sysvnp"):
+        c = cases.make_case(rt, p, {"layout": "C", "uplo": "L"}, {"n": n, "nrhs": 2}, V, nm,
+                            seed=4 + n)
+        A = cbkt.unpack(c["args"][4], nm, n, n, V)
+        bad = {3 + 2 * k: k for k in range(n)}
+        for mat, k in bad.items():
+            if rt == "posv":
+                A[mat, k, k] = -3.0
+            else:
+                A[mat, k, :k + 1] = 0.0
+                A[mat, :k + 1, k] = 0.0
+        c["args"][4] = cbkt.pack(A, V)
+        info_o = np.zeros(nm, np.int32)
+        info_g = np.zeros(nm, np.int32)
+        print(f"case reg info {rt}{p} n={n}", flush=True)
+        so, outo = cases.run(cbkt.oracle().call, c, info=info_o)
+        sg, outg = cases.run(_gpu().gpu_call, c, info=info_g)
+        assert so == sg == 0
+        assert np.array_equal(info_o, info_g)
+        want = np.zeros(nm, np.int32)
+        for mat, k in bad.items():
+            want[mat] = k + 1
+        assert np.array_equal(info_g, want), (info_g, want)
+        for a, b in zip(outo, outg):
+            assert cases.same_bits_nan(a, b)

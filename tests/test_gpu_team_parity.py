@@ -63,7 +63,7 @@ def _check_solution(case, xo, xg):
 @pytest.mark.parametrize("n", TEAM_N)
 def test_team_factor_and_solve(p, n):
     for i, (lay, uplo) in enumerate((("C", "L"), ("C", "U"), ("R", "L"), ("R", "U"))):
-        for V, nm in ((64, 130), (8, 13), (4, 6)):
+        for V, nm in ((64, 130), (32, 71), (8, 13), (4, 6)):
             for rt in ("potrf", "sytrfnp", "posv", "sysvnp", "potrs", "sytrsnp"):
                 for nrhs in ((0,) if rt in ("potrf", "sytrfnp") else (1, 4, 5, 24)):
                     dims = {"n": n} if nrhs == 0 else {"n": n, "nrhs": nrhs}
@@ -119,7 +119,9 @@ def test_team_info_codes(n):
         sg, outg = cases.run(g.gpu_call, c, info=info_g)
         assert so == sg == 0
         assert np.array_equal(info_o, info_g), (info_o, info_g)
-        assert all(info_g[m] > 0 for m in bad) and np.count_nonzero(info_g) == len(bad)
+        # the closed form: the first failing pivot is k, info = k + 1
+        assert all(info_g[m] == k + 1 for m, k in bad.items()), info_g
+        assert np.count_nonzero(info_g) == len(bad)
         assert cases.same_bits_nan(outo[0], outg[0])
         Bo = cbkt.unpack(outg[1], 70, n, 4, 64)
         B0 = cbkt.unpack(bp, 70, n, 4, 64)

@@ -27,7 +27,7 @@ def _gpu():
 def test_trsm_reg(p, m):
     combos = list(itertools.product("CR", "UL", "NT", "UN"))
     for i, (lay, uplo, tr, diag) in enumerate(combos):
-        for n, alpha, V, nm in ((3, 1.0, 64, 130), (4, -0.75, 8, 13), (1, 1.0, 2, 5), (m, 0.5, 16, 20)):
+        for n, alpha, V, nm in ((3, 1.0, 64, 130), (4, -0.75, 8, 13), (1, 1.0, 2, 5), (m, 0.5, 16, 20), (2, 1.0, 32, 71)):
             print(f"case trsm{p} {lay} L{uplo}{tr}{diag} m={m} n={n} alpha={alpha} V={V} nm={nm}", flush=True)
             c = cases.make_case("trsm", p, {"layout": lay, "side": "L", "uplo": uplo, "trans": tr, "diag": diag},
                                 {"m": m, "n": n, "alpha": alpha}, V, nm, seed=9300 + 17 * m + i + n,

@@ -23,7 +23,7 @@ def test_trsm_team_bitwise(p, m):
     for lay in ("C", "R"):
         for uplo, tr in (("L", "N"), ("U", "T"), ("U", "C"), ("L", "T"), ("U", "N")):
             for diag in ("U", "N"):
-                for n, alpha, V, nm in ((24, 1.0, 64, 130), (1, -0.5, 8, 13), (5, 2.0, 4, 6), (4, 1.0, 64, 64)):
+                for n, alpha, V, nm in ((24, 1.0, 64, 130), (1, -0.5, 8, 13), (5, 2.0, 4, 6), (4, 1.0, 64, 64), (3, -1.0, 32, 71)):
                     print(f"case trsm{p} L{uplo}{tr}{diag} {lay} m={m} n={n} alpha={alpha} V={V} nm={nm}",
                           flush=True)
                     c = cases.make_case("trsm", p, {"layout": lay, "side": "L", "uplo": uplo,
