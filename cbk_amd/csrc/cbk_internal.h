@@ -78,6 +78,14 @@ template <class T>
 hipError_t launch_reg_chol(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
                            Operand<T> B, int V, int nm, int *info);
 
+/* orders served by the potrf/posv/sytrfnp/sysvnp quad kernel (one matrix per
+ * 4 lanes, quad_sym.hip): FP64 24 and 32, FP32 32; any V; same contract as
+ * launch_chol_lane */
+bool quad_sym_supports(int n, bool fp64);
+template <class T>
+hipError_t launch_quad_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
+                           Operand<T> B, int V, int nm, int *info);
+
 /* orders served by the potrf/posv/sytrfnp/sysvnp team kernel */
 bool sym_team_supports(int n);
 
@@ -92,6 +100,13 @@ bool trsm_reg_supports(bool left, int m, bool fp64);
 template <class T>
 hipError_t launch_trsm_reg(bool up, bool tran, bool unit, int m, int n, T alpha, Operand<T> A,
                            Operand<T> B, int V, int nm);
+
+/* trsm, side L, forward sweep (op(A) lower) with one matrix per 4 lanes
+ * (quad_trsm.hip): FP64 m = 24, FP32 m = 32 */
+bool quad_trsm_supports(bool left, bool up, bool tran, int m, bool fp64);
+template <class T>
+hipError_t launch_quad_trsm(bool tran, bool unit, int m, int n, T alpha, Operand<T> A,
+                            Operand<T> B, int V, int nm);
 
 /* trsm, side L, forward sweep (op(A) lower), m in {16, 24, 32} */
 bool trsm_team_supports(bool left, bool up, bool tran, int m);

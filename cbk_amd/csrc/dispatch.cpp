@@ -16,6 +16,8 @@ hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand
      * faster than the team kernel at its orders 16 and 24 too) */
     if (fit && current_policy() != CBK_POLICY_LANE && reg_chol_supports(n, sizeof(T) == 8))
         return launch_reg_chol<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+    if (fit && current_policy() == CBK_POLICY_AUTO && quad_sym_supports(n, sizeof(T) == 8))
+        return launch_quad_sym<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     if (fit && current_policy() == CBK_POLICY_AUTO && sym_team_supports(n) && V >= 4)
         return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
@@ -29,6 +31,8 @@ hipError_t dispatch_trsm(bool left, bool up, bool tran, bool unit, int m, int n,
     const bool fit = offsets_fit(A, s, s, V) && offsets_fit(B, m, n, V);
     if (fit && current_policy() != CBK_POLICY_LANE && trsm_reg_supports(left, m, sizeof(T) == 8))
         return launch_trsm_reg<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
+    if (fit && current_policy() != CBK_POLICY_LANE && quad_trsm_supports(left, up, tran, m, sizeof(T) == 8))
+        return launch_quad_trsm<T>(tran, unit, m, n, alpha, A, B, V, nm);
     if (fit && current_policy() != CBK_POLICY_LANE && V >= 4 && trsm_team_supports(left, up, tran, m))
         return launch_trsm_team<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
     return launch_trsm_lane<T>(left, up, tran, unit, m, n, alpha, A, B, V, nm);
