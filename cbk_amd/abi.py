@@ -27,6 +27,7 @@ ROUTINES = {
 EXT = {
     "dsposv_compact": "cciipipipipzppii",
     "dsgels_compact": "cciiipipipippii",
+    "dsposv_compact_noalloc": "cciipipipippii",
 }
 
 # per-matrix info variants: the plain argument list plus a trailing device int[nm]

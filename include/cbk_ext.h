@@ -100,6 +100,29 @@ int dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap
 size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int ldap, int ldbp,
                                int ldxp, int V, int nm);
 
+/* ---- no device allocation ------------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: dsposv_compact_noalloc is dsposv_compact with the
+ * workspace taken from a static device array of the library instead of a
+ * caller's buffer (CBK_NOALLOC_SCRATCH_BYTES per GPU, held by each GPU the
+ * library runs on until the process ends). It calls no hipMalloc or hipFree.
+ * A call that would need more than that (the lane path: groups * V *
+ * (n*n + n*nrhs) * 4 bytes, e.g. n = 20, nrhs = 5 up to 16777 matrices)
+ * returns CBK_ERR_SCRATCH before any launch; shapes the fused kernels serve
+ * need none. Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (several streams or host threads at once) race and must be
+ * ordered by the caller. The array holds no state between calls.
+ * Codes: those of dsposv_compact for the same arguments (-11 iter NULL,
+ * -12 info NULL, -13 V, -14 nm, as their positions here), CBK_ERR_SCRATCH. */
+#define CBK_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+#define CBK_ERR_SCRATCH (-1001)
+int dsposv_compact_noalloc(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
+                           const double *bp, int ldbp, double *xp, int ldxp, int *iter,
+                           int *info, int V, int nm);
+/* the static array's size, CBK_NOALLOC_SCRATCH_BYTES */
+size_t cbk_noalloc_scratch_bytes(void);
+
 /* dsgels_compact: mixed-precision least squares min ||B - A X|| for every
  * matrix (trans 'N', n <= m <= 64, n + nrhs <= 64, full column rank). No LAPACK
  * routine has this contract; it follows dsposv's. The FP32 Householder QR of
