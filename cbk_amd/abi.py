@@ -28,6 +28,7 @@ EXT = {
     "dsposv_compact": "cciipipipipzppii",
     "dsgels_compact": "cciiipipipippii",
     "dsposv_compact_noalloc": "cciipipipippii",
+    "dssysvnp_compact": "cciipipipipzppii",
 }
 
 # per-matrix info variants: the plain argument list plus a trailing device int[nm]

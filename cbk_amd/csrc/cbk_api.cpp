@@ -265,7 +265,7 @@ constexpr int kItermax = 30; /* LAPACK dsposv's ITERMAX */
  * 32-bit offsets fit (offsets_fit) */
 bool mixed_uses_team(bool row, int n, int nrhs, int ldap, int ldbp, int ldxp, int V)
 {
-    return current_policy() == CBK_POLICY_AUTO && V >= 4 && mixed_team_supports(n, nrhs) &&
+    return current_policy() == CBK_POLICY_AUTO && V >= 4 && mixed_team_supports(n) &&
            offsets_fit_bytes(gstride(row, ldap, n, n, V), ldap, row, n, n, V, 8) &&
            offsets_fit_bytes(gstride(row, ldbp, n, nrhs, V), ldbp, row, n, nrhs, V, 8) &&
            offsets_fit_bytes(gstride(row, ldxp, n, nrhs, V), ldxp, row, n, nrhs, V, 8);
@@ -278,9 +278,12 @@ size_t mixed_worksize(bool row, int n, int nrhs, int ldap, int ldbp, int ldxp, i
     return groups * V * ((size_t)n * n + (size_t)n * nrhs) * sizeof(float);
 }
 
-int dsposv(char layout, char uplo, int n, int nrhs, double *ap, int ldap, const double *bp,
-           int ldbp, double *xp, int ldxp, void *work, size_t lwork, int *iter, int *info,
-           int V, int nm)
+/* dsposv_compact and dssysvnp_compact (include/cbk_ext.h): the FP32 factor
+ * (Cholesky, or ldl: the unpivoted LDL^T) refined in FP64, the FP64 factor on
+ * the matrices it fails */
+int mixed_sv(bool ldl, char layout, char uplo, int n, int nrhs, double *ap, int ldap,
+             const double *bp, int ldbp, double *xp, int ldxp, void *work, size_t lwork,
+             int *iter, int *info, int V, int nm)
 {
     const bool col = opt(layout, 'C'), row = opt(layout, 'R');
     const bool lo = opt(uplo, 'L'), up = opt(uplo, 'U');
@@ -306,9 +309,10 @@ int dsposv(char layout, char uplo, int n, int nrhs, double *ap, int ldap, const 
     const Operand<double> B = operand(const_cast<double *>(bp), ldbp, row, n, nrhs, V);
     const Operand<double> X = operand(xp, ldxp, row, n, nrhs, V);
     if (mixed_uses_team(row, n, nrhs, ldap, ldbp, ldxp, V))
-        return status(launch_mixed_posv_team(up, n, nrhs, A, B, X, V, nm, kItermax, iter, info));
-    return status(launch_mixed_posv_lane(up, n, nrhs, A, B, X, static_cast<float *>(work), V,
-                                         nm, kItermax, iter, info));
+        return status(
+            launch_mixed_posv_team(ldl, up, n, nrhs, A, B, X, V, nm, kItermax, iter, info));
+    return status(launch_mixed_posv_lane(ldl, up, n, nrhs, A, B, X, static_cast<float *>(work),
+                                         V, nm, kItermax, iter, info));
 }
 
 /* dsgels_compact (include/cbk_ext.h): the FP32-preconditioned refinement of
@@ -469,8 +473,16 @@ int dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap
                    const double *bp, int ldbp, double *xp, int ldxp, void *work,
                    size_t lwork, int *iter, int *info, int V, int nm)
 {
-    return dsposv(layout, uplo, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, work, lwork, iter,
-                  info, V, nm);
+    return mixed_sv(false, layout, uplo, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, work, lwork,
+                    iter, info, V, nm);
+}
+
+int dssysvnp_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
+                     const double *bp, int ldbp, double *xp, int ldxp, void *work,
+                     size_t lwork, int *iter, int *info, int V, int nm)
+{
+    return mixed_sv(true, layout, uplo, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, work, lwork,
+                    iter, info, V, nm);
 }
 
 int dsgels_compact(char layout, char trans, int m, int n, int nrhs, double *ap, int ldap,
@@ -486,6 +498,12 @@ size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int ldap
     (void)uplo;
     if (n < 0 || nrhs < 0 || nm < 0 || !vlen_ok(V)) return 0;
     return mixed_worksize(opt(layout, 'R'), n, nrhs, ldap, ldbp, ldxp, V, nm);
+}
+
+size_t dssysvnp_compact_worksize(char layout, char uplo, int n, int nrhs, int ldap, int ldbp,
+                                 int ldxp, int V, int nm)
+{
+    return dsposv_compact_worksize(layout, uplo, n, nrhs, ldap, ldbp, ldxp, V, nm);
 }
 
 CBK_ENTRY_POINTS(double, d)

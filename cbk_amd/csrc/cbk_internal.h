@@ -64,8 +64,9 @@ template <class T>
 hipError_t launch_gels_lane(bool tran, int m, int n, int nrhs, Operand<T> A, Operand<T> B,
                             T *tau, int V, int nm, int *info);
 
-/* mixed-precision posv, lane path: work holds groups*V*(n*n + n*nrhs) floats */
-hipError_t launch_mixed_posv_lane(bool up, int n, int nrhs, Operand<double> A,
+/* mixed-precision posv (ldl: sysvnp), lane path: work holds groups*V*(n*n +
+ * n*nrhs) floats */
+hipError_t launch_mixed_posv_lane(bool ldl, bool up, int n, int nrhs, Operand<double> A,
                                   Operand<double> B, Operand<double> X, float *work, int V,
                                   int nm, int itermax, int *iter, int *info);
 
@@ -127,11 +128,13 @@ template <class T>
 hipError_t launch_ormqr_team(bool left, bool tran, int m, int n, int k, Operand<T> A,
                              const T *tau, Operand<T> C, int V, int nm);
 
-/* orders and right-hand-side counts served by the fused mixed posv team kernel */
-bool mixed_team_supports(int n, int nrhs);
+/* orders served by the fused mixed posv / sysvnp team kernel (any nrhs, in
+ * blocks of 4 right-hand sides) */
+bool mixed_team_supports(int n);
 
-/* mixed posv, team path (no workspace); A is the operand as passed (uplo via up) */
-hipError_t launch_mixed_posv_team(bool up, int n, int nrhs, Operand<double> A,
+/* mixed posv (ldl: sysvnp), team path (no workspace); A is the operand as
+ * passed (uplo via up) */
+hipError_t launch_mixed_posv_team(bool ldl, bool up, int n, int nrhs, Operand<double> A,
                                   Operand<double> B, Operand<double> X, int V, int nm,
                                   int itermax, int *iter, int *info);
 

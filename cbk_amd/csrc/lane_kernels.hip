@@ -478,7 +478,10 @@ __global__ __launch_bounds__(kBlock) void gels_batch_lane_kernel(BatchArgs<T> p)
  * ld = n, the lane's slot), A and B are read through their views, X is
  * written. Same algorithm and codes as the team kernel (mixed_team_impl.h):
  * refinement until ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n) for every
- * column, FP64 fallback in place otherwise. */
+ * column, FP64 fallback in place otherwise. LDL: the unpivoted LDL^T of
+ * sysvnp in both precisions (the FP32 factor fails on a zero or non-finite
+ * pivot). */
+template <bool LDL>
 __global__ __launch_bounds__(kBlock) void mixed_posv_lane_kernel(
     bool up, int n, int nrhs, Operand<double> A, Operand<double> B, Operand<double> X,
     float *work, int V, long nlanes, int nm, int itermax, int *iter_out, int *info_out)
@@ -519,7 +522,11 @@ __global__ __launch_bounds__(kBlock) void mixed_posv_lane_kernel(
             for (int i = c; i < n; ++i)
                 S(i, c) = (float)Al(i, c);
         int finfo = 0;
-        chol_lane<float, false>(n, S, &finfo);
+        chol_lane<float, LDL>(n, S, &finfo);
+        if (LDL) {
+            for (int j = 0; j < n && finfo == 0; ++j)
+                if (!__builtin_isfinite(S(j, j))) finfo = j + 1;
+        }
         if (finfo != 0) {
             it_code = -3;
         }
@@ -527,7 +534,7 @@ __global__ __launch_bounds__(kBlock) void mixed_posv_lane_kernel(
             for (int q = 0; q < nrhs; ++q)
                 for (int i = 0; i < n; ++i)
                     W(i, q) = (float)Bv(i, q);
-            chol_solve_lane<float, false>(false, n, nrhs, S, W);
+            chol_solve_lane<float, LDL>(false, n, nrhs, S, W);
             for (int q = 0; q < nrhs; ++q)
                 for (int i = 0; i < n; ++i)
                     Xv(i, q) = (double)W(i, q);
@@ -560,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void mixed_posv_lane_kernel(
                     it_code = -2;
                     break;
                 }
-                chol_solve_lane<float, false>(false, n, nrhs, S, W);
+                chol_solve_lane<float, LDL>(false, n, nrhs, S, W);
                 for (int q = 0; q < nrhs; ++q)
                     for (int i = 0; i < n; ++i)
                         Xv(i, q) = Xv(i, q) + (double)W(i, q);
@@ -569,12 +576,12 @@ __global__ __launch_bounds__(kBlock) void mixed_posv_lane_kernel(
     }
     int dinfo = 0;
     if (!done) {
-        chol_lane<double, false>(n, Al, &dinfo);
+        chol_lane<double, LDL>(n, Al, &dinfo);
         if (dinfo == 0) {
             for (int q = 0; q < nrhs; ++q)
                 for (int i = 0; i < n; ++i)
                     Xv(i, q) = Bv(i, q);
-            chol_solve_lane<double, false>(up, n, nrhs, Av0, Xv);
+            chol_solve_lane<double, LDL>(up, n, nrhs, Av0, Xv);
         }
     }
     if (idx < nm) {
@@ -700,14 +707,19 @@ hipError_t launch_gels_lane(bool tran, int m, int n, int nrhs, Operand<T> A, Ope
     return hipGetLastError();
 }
 
-hipError_t launch_mixed_posv_lane(bool up, int n, int nrhs, Operand<double> A,
+hipError_t launch_mixed_posv_lane(bool ldl, bool up, int n, int nrhs, Operand<double> A,
                                   Operand<double> B, Operand<double> X, float *work, int V,
                                   int nm, int itermax, int *iter, int *info)
 {
     const long nl = nlanes_of(V, nm);
-    hipLaunchKernelGGL(mixed_posv_lane_kernel, dim3(nblocks(nl)), dim3(kBlock), 0,
-                       current_stream(), up, n, nrhs, A, B, X, work, V, nl, nm, itermax, iter,
-                       info);
+    if (ldl)
+        hipLaunchKernelGGL(mixed_posv_lane_kernel<true>, dim3(nblocks(nl)), dim3(kBlock), 0,
+                           current_stream(), up, n, nrhs, A, B, X, work, V, nl, nm, itermax, iter,
+                           info);
+    else
+        hipLaunchKernelGGL(mixed_posv_lane_kernel<false>, dim3(nblocks(nl)), dim3(kBlock), 0,
+                           current_stream(), up, n, nrhs, A, B, X, work, V, nl, nm, itermax, iter,
+                           info);
     return hipGetLastError();
 }
 

@@ -1,5 +1,5 @@
-/* mixed_team.hip -- host launcher of the fused mixed-precision posv team
- * kernel (mixed_team_impl.h); one order per translation unit,
+/* mixed_team.hip -- host launcher of the fused mixed-precision posv / sysvnp
+ * team kernel (mixed_team_impl.h); one order per translation unit,
  * mixed_team_<n>.hip. */
 
 #include "mixed_team_impl.h"
@@ -7,17 +7,14 @@
 namespace cbk {
 
 namespace mixedteam {
-extern template hipError_t launch_n<16>(const MixedArgs &);
-extern template hipError_t launch_n<24>(const MixedArgs &);
-extern template hipError_t launch_n<32>(const MixedArgs &);
+extern template hipError_t launch_n<16>(bool, const MixedArgs &);
+extern template hipError_t launch_n<24>(bool, const MixedArgs &);
+extern template hipError_t launch_n<32>(bool, const MixedArgs &);
 } /* namespace mixedteam */
 
-bool mixed_team_supports(int n, int nrhs)
-{
-    return (n == 16 || n == 24 || n == 32) && nrhs <= 4;
-}
+bool mixed_team_supports(int n) { return n == 16 || n == 24 || n == 32; }
 
-hipError_t launch_mixed_posv_team(bool up, int n, int nrhs, Operand<double> A,
+hipError_t launch_mixed_posv_team(bool ldl, bool up, int n, int nrhs, Operand<double> A,
                                   Operand<double> B, Operand<double> X, int V, int nm,
                                   int itermax, int *iter, int *info)
 {
@@ -48,9 +45,9 @@ hipError_t launch_mixed_posv_team(bool up, int n, int nrhs, Operand<double> A,
     p.iter = iter;
     p.info = info;
     switch (n) {
-    case 16: return mixedteam::launch_n<16>(p);
-    case 24: return mixedteam::launch_n<24>(p);
-    case 32: return mixedteam::launch_n<32>(p);
+    case 16: return mixedteam::launch_n<16>(ldl, p);
+    case 24: return mixedteam::launch_n<24>(ldl, p);
+    case 32: return mixedteam::launch_n<32>(ldl, p);
     default: return hipErrorInvalidValue;
     }
 }

@@ -3,6 +3,6 @@
 
 namespace cbk {
 namespace mixedteam {
-template hipError_t launch_n<16>(const MixedArgs &);
+template hipError_t launch_n<16>(bool, const MixedArgs &);
 }
 } /* namespace cbk */

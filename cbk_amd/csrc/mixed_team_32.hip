@@ -3,6 +3,6 @@
 
 namespace cbk {
 namespace mixedteam {
-template hipError_t launch_n<32>(const MixedArgs &);
+template hipError_t launch_n<32>(bool, const MixedArgs &);
 }
 } /* namespace cbk */

@@ -56,7 +56,7 @@ struct MixedArgs {
  * block would hold its CU's slots until its slowest matrix converged */
 constexpr int kTeamsPerBlock = 4;
 
-template <int N> hipError_t launch_n(const MixedArgs &args);
+template <int N> hipError_t launch_n(bool ldl, const MixedArgs &args);
 
 /* whether c holds in any lane of the calling lane's team (a wave-wide ballot,
  * then the team's 16 bits) */
@@ -90,7 +90,7 @@ __device__ __forceinline__ void team_max4(double (&x)[4], int t)
     x[3] = bcast<3>(k);
 }
 
-template <int N, int NRB>
+template <int N, int NRB, bool LDL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N <= 16 ? 4 : 2, N <= 16 ? 4 : 2))) void
 mixed_posv_kernel(MixedArgs p)
 {
@@ -115,6 +115,7 @@ mixed_posv_kernel(MixedArgs p)
     const __amdgpu_buffer_rsrc_t rb = group_rsrc(p.b + g * p.b_gs, group_range(groups_left, p.b_gs, SZ));
     const __amdgpu_buffer_rsrc_t rx = group_rsrc(p.x + g * p.x_gs, group_range(groups_left, p.x_gs, SZ));
     const unsigned a_col = (unsigned)p.a_sj * V * SZ, a_rowb = (unsigned)p.a_si * V * SZ;
+    const unsigned b_colb = (unsigned)p.b_sj * V * SZ, x_colb = (unsigned)p.x_sj * V * SZ;
     unsigned a_row[R], a_cbase[R], b_row[R], x_row[R];
     sfor<0, R>([&](auto r) {
         const int i = t + TS * r;
@@ -123,11 +124,6 @@ mixed_posv_kernel(MixedArgs p)
         a_cbase[r] = in ? ((unsigned)(i * p.a_sj) * V + v) * SZ : kOOB; /* (c, i): + c*a_rowb */
         b_row[r] = in ? ((unsigned)(i * p.b_si) * V + v) * SZ : kOOB;
         x_row[r] = in ? ((unsigned)(i * p.x_si) * V + v) * SZ : kOOB;
-    });
-    unsigned bcol[NRB], xcol[NRB];
-    sfor<0, NRB>([&](auto q) {
-        bcol[q] = (q < p.nrhs) ? q * (unsigned)p.b_sj * V * SZ : kOOB;
-        xcol[q] = (q < p.nrhs) ? q * (unsigned)p.x_sj * V * SZ : kOOB;
     });
 
     /* full symmetric rows of A in FP64: (i, c) from the lower view's (i, c)
@@ -162,145 +158,183 @@ mixed_posv_kernel(MixedArgs p)
     const bool a_over = team_any(over);
 
     int iter = 0;
-    bool done = false;
-    bool ready = false; /* the FP32 factor is in LDS */
+    bool failed = false;
     float *L32 = reinterpret_cast<float *>(lds + team * TW);
-    float dinv32[R];
+    float dsc32[R];
     if (!a_over) {
+        /* the FP32 factor: Cholesky with the hardware reciprocal square root,
+         * or the unpivoted LDL^T (which fails on a zero or non-finite D) */
         float a32[R][N], dg32[R];
         sfor<0, R>([&](auto r) {
             sfor<0, cmin(TS * (r + 1), N)>([&](auto c) { a32[r][c] = (float)a64[r][c]; });
         });
-        const int finfo = chol_factor_fast<N>(a32, dg32, t);
-        if (finfo != 0) {
-            iter = -3;
+        int finfo;
+        if constexpr (LDL) {
+            finfo = chol_factor<float, N, true>(a32, dg32, t);
+            bool bad = false;
+            sfor<0, R>([&](auto r) {
+                if (t + TS * r < N) bad |= !__builtin_isfinite(dg32[r]);
+            });
+            if (team_any(bad)) finfo = 1;
         }
         else {
-            chol_prepare_cols<N>(a32, dg32, dinv32, L32, t);
-            ready = true;
+            finfo = chol_factor_fast<N>(a32, dg32, t);
+        }
+        if (finfo != 0) {
+            iter = -3;
+            failed = true;
+        }
+        else {
+            chol_prepare_cols<N, LDL>(a32, dg32, dsc32, L32, t);
         }
     }
     else {
         iter = -2;
-    }
-    /* B is read after the factor, whose registers it would otherwise share;
-     * an overflowing B (dlag2s) sends the matrix to FP64 like an overflowing
-     * A, whatever the factor did (dsposv tests B first) */
-    double b64[R][NRB];
-    sfor<0, R>([&](auto r) {
-        sfor<0, NRB>([&](auto q) { b64[r][q] = bload(0.0, rb, b_row[r], bcol[q]); });
-    });
-    bool b_over = false;
-    sfor<0, R>([&](auto r) {
-        sfor<0, NRB>([&](auto q) { b_over |= __builtin_fabs(b64[r][q]) > 3.4028234663852886e38; });
-    });
-    if (team_any(b_over)) {
-        iter = -2;
-        ready = false;
+        failed = true;
     }
 
-    double x64[R][NRB];
-    if (ready) {
-        {
-            float w[R][NRB];
-            sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { w[r][q] = (float)b64[r][q]; }); });
-            chol_solve_cols<N, NRB>(dinv32, w, L32, t);
-            sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { x64[r][q] = (double)w[r][q]; }); });
-            for (int it = 0;; ++it) {
-                /* r = b - A x, FP64 */
-                double res[R][NRB];
-                sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { res[r][q] = b64[r][q]; }); });
-                sfor<0, R>([&](auto r) { dpp_ready_from<0>(x64[r]); });
-                sfor<0, N>([&](auto c) {
-                    sfor<0, NRB>([&](auto q) {
-                        sfor<0, R>([&](auto r) { fnma_bcr<c % TS>(res[r][q], x64[c / TS][q], a64[r][c]); });
-                    });
-                });
-                /* every column: ||r||_inf <= ||x||_inf * cte, tested per row
-                 * against the team's ||x||_inf (a NaN fails either way) */
-                double xm[NRB];
-                sfor<0, NRB>([&](auto q) {
-                    xm[q] = 0.0;
-                    sfor<0, R>([&](auto r) {
-                        if (t + TS * r < N) xm[q] = fmax_nan(xm[q], __builtin_fabs(x64[r][q]));
-                    });
-                });
-                team_max4(xm, t);
-                bool fail = false;
-                sfor<0, NRB>([&](auto q) {
-                    sfor<0, R>([&](auto r) {
-                        if (q < p.nrhs && t + TS * r < N)
-                            fail = fail || !(__builtin_fabs(res[r][q]) <= xm[q] * cte);
-                    });
-                });
-                const bool conv = !team_any(fail);
-                if (conv) {
-                    iter = it;
-                    done = true;
-                    break;
-                }
-                if (it == p.itermax) {
-                    iter = -p.itermax - 1;
-                    break;
-                }
-                bool rover = false;
-                sfor<0, R>([&](auto r) {
-                    sfor<0, NRB>([&](auto q) {
-                        rover |= __builtin_fabs(res[r][q]) > 3.4028234663852886e38;
-                        w[r][q] = (float)res[r][q];
-                    });
-                });
-                if (team_any(rover)) {
-                    iter = -2;
-                    break;
-                }
-                chol_solve_cols<N, NRB>(dinv32, w, L32, t);
-                sfor<0, R>([&](auto r) {
-                    sfor<0, NRB>([&](auto q) { x64[r][q] = x64[r][q] + (double)w[r][q]; });
-                });
-            }
+    /* right-hand sides in blocks of NRB, each refined until its columns pass
+     * (iter: the most steps a block took); B is read after the factor, whose
+     * registers it would otherwise share */
+    for (int q0 = 0; q0 < p.nrhs; q0 += NRB) {
+        unsigned bcol[NRB], xcol[NRB];
+        sfor<0, NRB>([&](auto q) {
+            bcol[q] = (q0 + q < p.nrhs) ? (q0 + q) * b_colb : kOOB;
+            xcol[q] = (q0 + q < p.nrhs) ? (q0 + q) * x_colb : kOOB;
+        });
+        double b64[R][NRB];
+        sfor<0, R>([&](auto r) {
+            sfor<0, NRB>([&](auto q) { b64[r][q] = bload(0.0, rb, b_row[r], bcol[q]); });
+        });
+        /* an overflowing B (dlag2s) sends the matrix to FP64 like an
+         * overflowing A, whatever the factor did (dsposv tests B first) */
+        bool b_over = false;
+        sfor<0, R>([&](auto r) {
+            sfor<0, NRB>([&](auto q) { b_over |= __builtin_fabs(b64[r][q]) > 3.4028234663852886e38; });
+        });
+        if (team_any(b_over)) {
+            iter = -2;
+            failed = true;
         }
+        if (failed) continue; /* the team's lanes alike */
+        double x64[R][NRB];
+        float w[R][NRB];
+        sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { w[r][q] = (float)b64[r][q]; }); });
+        chol_solve_cols<N, NRB>(dsc32, w, L32, t);
+        sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { x64[r][q] = (double)w[r][q]; }); });
+        int it_blk = 0;
+        for (int it = 0;; ++it) {
+            /* r = b - A x, FP64 */
+            double res[R][NRB];
+            sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { res[r][q] = b64[r][q]; }); });
+            sfor<0, R>([&](auto r) { dpp_ready_from<0>(x64[r]); });
+            sfor<0, N>([&](auto c) {
+                sfor<0, NRB>([&](auto q) {
+                    sfor<0, R>([&](auto r) { fnma_bcr<c % TS>(res[r][q], x64[c / TS][q], a64[r][c]); });
+                });
+            });
+            /* every column: ||r||_inf <= ||x||_inf * cte, tested per row
+             * against the team's ||x||_inf (a NaN fails either way) */
+            double xm[NRB];
+            sfor<0, NRB>([&](auto q) {
+                xm[q] = 0.0;
+                sfor<0, R>([&](auto r) {
+                    if (t + TS * r < N) xm[q] = fmax_nan(xm[q], __builtin_fabs(x64[r][q]));
+                });
+            });
+            team_max4(xm, t);
+            bool fail = false;
+            sfor<0, NRB>([&](auto q) {
+                sfor<0, R>([&](auto r) {
+                    if (q0 + q < p.nrhs && t + TS * r < N)
+                        fail = fail || !(__builtin_fabs(res[r][q]) <= xm[q] * cte);
+                });
+            });
+            if (!team_any(fail)) {
+                it_blk = it;
+                break;
+            }
+            if (it == p.itermax) {
+                iter = -p.itermax - 1;
+                failed = true;
+                break;
+            }
+            bool rover = false;
+            sfor<0, R>([&](auto r) {
+                sfor<0, NRB>([&](auto q) {
+                    rover |= __builtin_fabs(res[r][q]) > 3.4028234663852886e38;
+                    w[r][q] = (float)res[r][q];
+                });
+            });
+            if (team_any(rover)) {
+                iter = -2;
+                failed = true;
+                break;
+            }
+            chol_solve_cols<N, NRB>(dsc32, w, L32, t);
+            sfor<0, R>([&](auto r) {
+                sfor<0, NRB>([&](auto q) { x64[r][q] = x64[r][q] + (double)w[r][q]; });
+            });
+        }
+        if (failed) continue;
+        iter = it_blk > iter ? it_blk : iter;
+        sfor<0, R>([&](auto r) {
+            sfor<0, NRB>([&](auto q) { bstore(x64[r][q], rx, x_row[r], xcol[q]); });
+        });
     }
 
     int dinfo = 0;
-    if (!done) {
-        /* FP64 fallback: dposv on A in place; A receives the factor */
+    if (__builtin_amdgcn_ballot_w64(failed) != 0) {
+        /* FP64 fallback in place on the failed matrices: A receives the factor
+         * (bit-identical to the reference's potrf / sytrfnp), X the solution */
         double a[R][N], dg[R], dinv[R];
         sfor<0, R>([&](auto r) {
             sfor<0, cmin(TS * (r + 1), N)>([&](auto c) { a[r][c] = a64[r][c]; });
         });
-        dinfo = chol_factor<double, N, false>(a, dg, t);
+        dinfo = chol_factor<double, N, LDL>(a, dg, t);
         sfor<0, R>([&](auto r) {
             const int i = t + TS * r;
             sfor<0, cmin(TS * (r + 1), N)>([&](auto c) {
-                const unsigned off = (c < TS * r || c < i) ? a_row[r] : kOOB;
+                const unsigned off = (failed && (c < TS * r || c < i)) ? a_row[r] : kOOB;
                 bstore(a[r][c], ra, off, c * a_col);
             });
-            const unsigned doff = (a_row[r] == kOOB) ? kOOB : a_row[r] + (unsigned)i * a_col;
+            const unsigned doff = (!failed || a_row[r] == kOOB) ? kOOB : a_row[r] + (unsigned)i * a_col;
             bstore(dg[r], ra, doff, 0);
         });
-        if (dinfo == 0) {
-            chol_prepare<double, N, false>(a, dg, dinv, lds + team * TW, t);
-            sfor<0, R>([&](auto r) { sfor<0, NRB>([&](auto q) { x64[r][q] = b64[r][q]; }); });
-            chol_solve<double, N, NRB, false>(a, dinv, x64, lds + team * TW, t);
+        chol_prepare<double, N, LDL>(a, dg, dinv, lds + team * TW, t);
+        const bool put = failed && dinfo == 0;
+        for (int q0 = 0; q0 < p.nrhs; q0 += NRB) {
+            unsigned bcol[NRB], xcol[NRB];
+            sfor<0, NRB>([&](auto q) {
+                bcol[q] = (q0 + q < p.nrhs) ? (q0 + q) * b_colb : kOOB;
+                xcol[q] = (q0 + q < p.nrhs) ? (q0 + q) * x_colb : kOOB;
+            });
+            double xb[R][NRB];
+            sfor<0, R>([&](auto r) {
+                sfor<0, NRB>([&](auto q) { xb[r][q] = bload(0.0, rb, b_row[r], bcol[q]); });
+            });
+            chol_solve<double, N, NRB, LDL>(a, dinv, xb, lds + team * TW, t);
+            sfor<0, R>([&](auto r) {
+                sfor<0, NRB>([&](auto q) { bstore(xb[r][q], rx, put ? x_row[r] : kOOB, xcol[q]); });
+            });
         }
+        if (!failed) dinfo = 0;
     }
     if (mat < p.nm && t == 0) {
         p.iter[mat] = iter;
         p.info[mat] = dinfo;
     }
-    if (dinfo == 0) {
-        sfor<0, R>([&](auto r) {
-            sfor<0, NRB>([&](auto q) { bstore(x64[r][q], rx, x_row[r], xcol[q]); });
-        });
-    }
 }
 
-template <int N> hipError_t launch_n(const MixedArgs &args)
+template <int N> hipError_t launch_n(bool ldl, const MixedArgs &args)
 {
     const unsigned blocks = (unsigned)((args.nmat + kTeamsPerBlock - 1) / kTeamsPerBlock);
-    hipLaunchKernelGGL((mixed_posv_kernel<N, 4>), dim3(xcd_grid(blocks)), dim3(64), 0,
-                       current_stream(), args);
+    if (ldl)
+        hipLaunchKernelGGL((mixed_posv_kernel<N, 4, true>), dim3(xcd_grid(blocks)), dim3(64), 0,
+                           current_stream(), args);
+    else
+        hipLaunchKernelGGL((mixed_posv_kernel<N, 4, false>), dim3(xcd_grid(blocks)), dim3(64), 0,
+                           current_stream(), args);
     return hipGetLastError();
 }
 

@@ -304,19 +304,27 @@ template <int N> constexpr int cols_at(int c)
 /* floats of the copy */
 template <int N> constexpr int cols_lds() { return cols_at<N>(N); }
 
-template <int N>
+/* LDL: a holds the unit lower factor of an LDL^T and dg its D; the copy is
+ * then L itself, and dsc = 1/D (the x = dsc * v step of chol_solve_cols).
+ * Cholesky: the columns scaled by 1/L(c,c) and dsc = 1/L(i,i)^2. */
+template <int N, bool LDL = false>
 __device__ __forceinline__ void chol_prepare_cols(const float (&a)[rows_per_lane<N>()][N],
                                                   const float (&dg)[rows_per_lane<N>()],
-                                                  float (&dinv)[rows_per_lane<N>()], float *L,
+                                                  float (&dsc)[rows_per_lane<N>()], float *L,
                                                   int t)
 {
     constexpr int R = rows_per_lane<N>();
-    sfor<0, R>([&](auto r) { dinv[r] = 1.0f / dg[r]; });
+    float dinv[R];
+    sfor<0, R>([&](auto r) {
+        dinv[r] = 1.0f / dg[r];
+        dsc[r] = LDL ? dinv[r] : dinv[r] * dinv[r];
+    });
     sfor<0, N>([&](auto c) {
-        const float dc = bcast<c % TS>(dinv[c / TS]);
+        const float dc = LDL ? 1.0f : bcast<c % TS>(dinv[c / TS]);
         sfor<c / TS, R>([&](auto r) {
             const int row = t + TS * r;
-            L[cols_at<N>(c) + TS * (r - c / TS) + t] = (c < row && row < N) ? a[r][c] * dc : 0.0f;
+            L[cols_at<N>(c) + TS * (r - c / TS) + t] =
+                (c < row && row < N) ? (LDL ? a[r][c] : a[r][c] * dc) : 0.0f;
         });
     });
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -325,7 +333,7 @@ __device__ __forceinline__ void chol_prepare_cols(const float (&a)[rows_per_lane
 }
 
 template <int N, int NRB>
-__device__ __forceinline__ void chol_solve_cols(const float (&dinv)[rows_per_lane<N>()],
+__device__ __forceinline__ void chol_solve_cols(const float (&dsc)[rows_per_lane<N>()],
                                                 float (&b)[rows_per_lane<N>()][NRB],
                                                 const float *L, int t)
 {
@@ -352,10 +360,9 @@ __device__ __forceinline__ void chol_solve_cols(const float (&dinv)[rows_per_lan
         });
         __builtin_amdgcn_sched_barrier(0);
     });
-    /* x starts from D^-2 v */
+    /* x starts from D^-2 v (Cholesky; LDL^T: D^-1 v) */
     sfor<0, R>([&](auto r) {
-        const float s = dinv[r] * dinv[r];
-        sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * s; });
+        sfor<0, NRB>([&](auto q) { b[r][q] = b[r][q] * dsc[r]; });
     });
     /* backward: column i = t + 16r starts at cols_at(i) with row 16r */
     int base[R];

@@ -94,11 +94,27 @@ int dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap
                    size_t lwork, int *iter, int *info, int V, int nm);
 
 /* bytes of workspace dsposv_compact needs for this call (0 if none): 0 when
- * a fused kernel serves it (n in {16, 24, 32}, nrhs <= 4, V >= 4, and leading
+ * a fused kernel serves it (n in {16, 24, 32}, V >= 4, and leading
  * dimensions small enough for the kernels' 32-bit offsets -- below about
  * 2^31 / (64 * 8 * n) at V = 64), else groups * V * (n*n + n*nrhs) * 4 */
 size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int ldap, int ldbp,
                                int ldxp, int V, int nm);
+
+/* dssysvnp_compact: dsposv_compact's contract with the unpivoted LDL^T of
+ * sysvnp_compact in place of the Cholesky factor. No LAPACK routine has this
+ * contract; the reference's sysvnp runs in one precision throughout
+ * (include/cbk.h:274-279).
+ * The FP32 LDL^T fails (iter = -3) on a zero or non-finite D; the FP64
+ * fallback is dsysvnp_compact, A then holding its factor (unit L strictly
+ * below the diagonal, D on it, bit-identical to dsytrfnp_compact), and info
+ * its code: j (1-based) of the first exact zero pivot D(j) of the FP64
+ * factor (X of that matrix unspecified). Same codes and workspace as
+ * dsposv_compact; dssysvnp_compact_worksize equals dsposv_compact_worksize. */
+int dssysvnp_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
+                     const double *bp, int ldbp, double *xp, int ldxp, void *work,
+                     size_t lwork, int *iter, int *info, int V, int nm);
+size_t dssysvnp_compact_worksize(char layout, char uplo, int n, int nrhs, int ldap, int ldbp,
+                                 int ldxp, int V, int nm);
 
 /* ---- no device allocation ------------------------------------------------
  * No counterpart in the reference. For callers that cannot allocate device

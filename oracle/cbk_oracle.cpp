@@ -20,6 +20,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 namespace {
@@ -1029,17 +1030,24 @@ int oracle_sgels_batch(char trans, int nprob, const oracle_gels_problem *desc, f
     return gels_batch<float>(trans, nprob, desc, ap, bp, info);
 }
 
-/* LAPACK 3.12 dsposv.f, per matrix, on the compact layout. The inner single
- * precision factor and solves are this file's restatement of the reference's
- * spotrf/spotrs lane kernels. Per-matrix codes follow dsposv's ITER: 0..30 the
- * refinement steps taken; -3 the single factor failed; -31 refinement did not
- * converge in 30 steps. In both negative cases the double dposv ran. */
-int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
-                          const double *bp, int ldbp, double *xp, int ldxp, void *work,
-                          size_t lwork, int *iter, int *info, int V, int nm)
+} /* extern "C" */
+
+namespace {
+
+/* LAPACK 3.12 dsposv.f, per matrix, on the compact layout; LDL: the same
+ * algorithm with the unpivoted LDL^T of sysvnp in both precisions. The inner
+ * single precision factor and solves are this file's restatement of the
+ * reference's spotrf/spotrs (ssytrfnp/ssytrsnp) lane kernels. Per-matrix
+ * codes follow dsposv's ITER: 0..30 the refinement steps taken; -2 B, A or a
+ * residual overflows single precision (dlag2s/dlat2s, tested on B first);
+ * -3 the single factor failed (LDL: also a non-finite D); -31 refinement did
+ * not converge in 30 steps. In the negative cases the double dposv (dsysvnp)
+ * ran on A in place. A residual or iterate holding a NaN never passes the
+ * convergence test. */
+template <bool LDL>
+int mixed_sv(char layout, char uplo, int n, int nrhs, double *ap, int ldap, const double *bp,
+             int ldbp, double *xp, int ldxp, int *iter, int *info, int V, int nm)
 {
-    (void)work;
-    (void)lwork;
     const bool col = opt(layout, 'C'), row = opt(layout, 'R');
     const bool lo = opt(uplo, 'L'), up = opt(uplo, 'U');
     if (!col && !row) return -1;
@@ -1060,6 +1068,7 @@ int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, i
                  sx = group_stride(row, ldxp, n, nrhs, V);
     const double eps = std::ldexp(1.0, -53); /* dlamch('Epsilon') */
     const double cte_scale = eps * std::sqrt((double)n);
+    const double rmax = (double)std::numeric_limits<float>::max(); /* slamch('O') */
     const int itermax = 30;
     for_each_lane(nm, V, [&](long g, long v) {
         const long idx = g * V + v;
@@ -1068,6 +1077,7 @@ int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, i
         if (up) A = A.tr();
         const LV<double> B = make_view(const_cast<double *>(bp) + g * sb + v, row, ldbp, V);
         const LV<double> X = make_view(xp + g * sx + v, row, ldxp, V);
+        auto over = [&](double x) { return x < -rmax || x > rmax; };
         /* ||A||_inf of the symmetric matrix (dlansy 'I') */
         double anrm = 0;
         for (long i = 0; i < n; ++i) {
@@ -1077,70 +1087,135 @@ int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, i
             anrm = s > anrm ? s : anrm;
         }
         const double cte = anrm * cte_scale;
+        bool b_over = false, a_over = false;
+        for (long c = 0; c < nrhs; ++c)
+            for (long i = 0; i < n; ++i)
+                b_over |= over(B(i, c));
+        for (long j = 0; j < n; ++j)
+            for (long i = j; i < n; ++i)
+                a_over |= over(A(i, j));
         /* single copies: column-major n x n (lower) and n x nrhs, V = 1 */
         std::vector<float> SA((size_t)n * n, 0.f), SX((size_t)n * nrhs);
         const LV<float> SAv{SA.data(), 1, n, 1};
-        for (long j = 0; j < n; ++j)
-            for (long i = j; i < n; ++i)
-                SA[j * n + i] = (float)A(i, j);
-        int finfo = 0;
-        potrf_block<float>(0, n, n, SAv, &finfo);
         std::vector<double> R((size_t)n * nrhs);
         int it = 0;
         bool done = false;
-        if (finfo != 0) {
-            it = -3;
+        if (b_over || a_over) {
+            it = -2;
         }
         else {
-            for (long c = 0; c < nrhs; ++c)
-                for (long i = 0; i < n; ++i)
-                    SX[c * n + i] = (float)B(i, c);
-            potrs_lane<float>(false, false, n, nrhs, SA.data(), n, SX.data(), n, 1);
-            for (long c = 0; c < nrhs; ++c)
-                for (long i = 0; i < n; ++i)
-                    X(i, c) = (double)SX[c * n + i];
-            for (int step = 0; step <= itermax && !done; ++step) {
-                sym_residual(n, nrhs, A, X, B, R);
-                bool conv = true;
-                for (long c = 0; c < nrhs; ++c) {
-                    double xn = 0, rn = 0;
-                    for (long i = 0; i < n; ++i) {
-                        xn = std::fmax(xn, std::fabs(X(i, c)));
-                        rn = std::fmax(rn, std::fabs(R[c * n + i]));
-                    }
-                    if (!(rn <= xn * cte)) conv = false;
-                }
-                if (conv) {
-                    it = step;
-                    done = true;
-                    break;
-                }
-                if (step == itermax) break;
-                for (size_t q = 0; q < R.size(); ++q)
-                    SX[q] = (float)R[q];
-                potrs_lane<float>(false, false, n, nrhs, SA.data(), n, SX.data(), n, 1);
+            for (long j = 0; j < n; ++j)
+                for (long i = j; i < n; ++i)
+                    SA[j * n + i] = (float)A(i, j);
+            int finfo = 0;
+            if (LDL) {
+                sytrfnp_block<float>(0, n, n, SAv, &finfo);
+                for (long j = 0; j < n && finfo == 0; ++j)
+                    if (!std::isfinite(SA[j * n + j])) finfo = (int)j + 1;
+            }
+            else {
+                potrf_block<float>(0, n, n, SAv, &finfo);
+            }
+            auto solve32 = [&]() {
+                if (LDL)
+                    sytrsnp_lane<float>(false, false, n, nrhs, SA.data(), n, SX.data(), n, 1);
+                else
+                    potrs_lane<float>(false, false, n, nrhs, SA.data(), n, SX.data(), n, 1);
+            };
+            if (finfo != 0) {
+                it = -3;
+            }
+            else {
                 for (long c = 0; c < nrhs; ++c)
                     for (long i = 0; i < n; ++i)
-                        X(i, c) = X(i, c) + (double)SX[c * n + i];
+                        SX[c * n + i] = (float)B(i, c);
+                solve32();
+                for (long c = 0; c < nrhs; ++c)
+                    for (long i = 0; i < n; ++i)
+                        X(i, c) = (double)SX[c * n + i];
+                for (int step = 0;; ++step) {
+                    sym_residual(n, nrhs, A, X, B, R);
+                    bool conv = true;
+                    for (long c = 0; c < nrhs; ++c) {
+                        double xn = 0, rn = 0;
+                        for (long i = 0; i < n; ++i) {
+                            xn = std::fmax(xn, std::fabs(X(i, c)));
+                            rn = std::fmax(rn, std::fabs(R[c * n + i]));
+                            if (std::isnan(X(i, c)) || std::isnan(R[c * n + i])) conv = false;
+                        }
+                        if (!(rn <= xn * cte)) conv = false;
+                    }
+                    if (conv) {
+                        it = step;
+                        done = true;
+                        break;
+                    }
+                    if (step == itermax) {
+                        it = -itermax - 1;
+                        break;
+                    }
+                    bool r_over = false;
+                    for (size_t q = 0; q < R.size(); ++q) r_over |= over(R[q]);
+                    if (r_over) {
+                        it = -2;
+                        break;
+                    }
+                    for (size_t q = 0; q < R.size(); ++q)
+                        SX[q] = (float)R[q];
+                    solve32();
+                    for (long c = 0; c < nrhs; ++c)
+                        for (long i = 0; i < n; ++i)
+                            X(i, c) = X(i, c) + (double)SX[c * n + i];
+                }
             }
-            if (!done) it = -itermax - 1;
         }
         if (!done) {
-            /* fallback: the double dposv on A, X := B first */
+            /* fallback: the double dposv (dsysvnp) on A, X := B first */
             int dinfo = 0;
-            potrf_block<double>(0, n, n, A, &dinfo);
+            if (LDL)
+                sytrfnp_block<double>(0, n, n, A, &dinfo);
+            else
+                potrf_block<double>(0, n, n, A, &dinfo);
             info[idx] = dinfo;
             if (dinfo == 0) {
                 for (long c = 0; c < nrhs; ++c)
                     for (long i = 0; i < n; ++i)
                         X(i, c) = B(i, c);
-                potrs_lane<double>(row, up, n, nrhs, ap + g * sa + v, ldap, xp + g * sx + v,
-                                   ldxp, V);
+                if (LDL)
+                    sytrsnp_lane<double>(row, up, n, nrhs, ap + g * sa + v, ldap,
+                                         xp + g * sx + v, ldxp, V);
+                else
+                    potrs_lane<double>(row, up, n, nrhs, ap + g * sa + v, ldap, xp + g * sx + v,
+                                       ldxp, V);
             }
         }
         iter[idx] = it;
     });
     return 0;
+}
+
+} /* namespace */
+
+extern "C" {
+
+int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
+                          const double *bp, int ldbp, double *xp, int ldxp, void *work,
+                          size_t lwork, int *iter, int *info, int V, int nm)
+{
+    (void)work;
+    (void)lwork;
+    return mixed_sv<false>(layout, uplo, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, iter, info, V,
+                           nm);
+}
+
+int oracle_dssysvnp_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
+                            const double *bp, int ldbp, double *xp, int ldxp, void *work,
+                            size_t lwork, int *iter, int *info, int V, int nm)
+{
+    (void)work;
+    (void)lwork;
+    return mixed_sv<true>(layout, uplo, n, nrhs, ap, ldap, bp, ldbp, xp, ldxp, iter, info, V,
+                          nm);
 }
 
 } /* extern "C" */

@@ -76,11 +76,21 @@ ORACLE_DECLS(float, s)
  * counterpart; LAPACK 3.12 dsposv.f): single-precision Cholesky, refinement with
  * double residuals, ITERMAX = 30, convergence when ||r||_inf <= ||x||_inf *
  * ||A||_inf * eps * sqrt(n) for every column, fallback to the double-precision
- * dposv otherwise. xp receives X; ap keeps A unless the fallback ran, in which
- * case it holds the double Cholesky factor. iter and info are per matrix. */
+ * dposv otherwise (-2 when B, A or a residual overflows single precision, -3
+ * when the single factor fails, -31 without convergence). xp receives X; ap
+ * keeps A unless the fallback ran, in which case it holds the double Cholesky
+ * factor. iter and info are per matrix. */
 int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
                           const double *bp, int ldbp, double *xp, int ldxp, void *work,
                           size_t lwork, int *iter, int *info, int V, int nm);
+
+/* dsposv's algorithm with the unpivoted LDL^T of sysvnp in both precisions
+ * (no reference counterpart, no LAPACK routine): the single factor also fails
+ * on a non-finite D; the fallback is the double dsysvnp, after which ap holds
+ * the double LDL^T factor. The library's dssysvnp_compact. */
+int oracle_dssysvnp_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap,
+                            const double *bp, int ldbp, double *xp, int ldxp, void *work,
+                            size_t lwork, int *iter, int *info, int V, int nm);
 
 /* Mixed-precision least squares on the compact layout (no reference
  * counterpart): single precision QR as the preconditioner of refinement on the

@@ -1,4 +1,5 @@
-"""Mixed-precision dposv (dsposv_compact) on the GPU against the FP64 oracle.
+"""Mixed-precision dposv (dsposv_compact) and sysvnp (dssysvnp_compact) on the
+GPU against the FP64 oracle.
 
 Checks, per matrix:
   - backward error max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
@@ -8,9 +9,16 @@ Checks, per matrix:
   - refinement succeeded (iter >= 0) and A is untouched, bit for bit;
   - forced fallback (cond ~ 1e12 > 1/eps_32): iter < 0, A holds the FP64
     Cholesky factor bit-identical to the oracle's potrf, X matches dposv;
-  - a non-SPD matrix: info matches the oracle's FP64 dpotrf code.
-Both the fused team kernel (n in {16,24,32}, nrhs <= 4) and the workspace lane
-kernel (other shapes, or the lane policy) are exercised.
+  - a non-SPD matrix: info matches the oracle's FP64 dpotrf code;
+  - the same for dssysvnp_compact on symmetric indefinite matrices (the FP64
+    LDL^T factor bit-identical to the oracle's sytrfnp, info its first zero
+    pivot);
+  - every ITER code from a construction that fixes it (-2: B, A entry beyond
+    FLT_MAX; -3: a diagonal that is 0 in FP32, for sysvnp also a D whose FP32
+    reciprocal overflows), equal to the oracle's, X then bit-identical to the
+    oracle's FP64 solve.
+Both the fused team kernel (n in {16,24,32}, any nrhs in blocks of 4) and the
+workspace lane kernel (other shapes, or the lane policy) are exercised.
 """
 
 import numpy as np
@@ -31,7 +39,7 @@ def spd_logspace(r, nm, n, kmax_choices=(2, 4, 6)):
     return 0.5 * (A + np.swapaxes(A, 1, 2))
 
 
-def run_dsposv(p_call, A, B, V, uplo="L", layout="C"):
+def run_dsposv(p_call, A, B, V, uplo="L", layout="C", name="dsposv_compact"):
     nm, n, _ = A.shape
     nrhs = B.shape[2]
     row = layout == "R"
@@ -43,7 +51,7 @@ def run_dsposv(p_call, A, B, V, uplo="L", layout="C"):
     iters = np.zeros(nm, np.int32)
     info = np.zeros(nm, np.int32)
     if p_call == "oracle":
-        st = cbkt.oracle().call_raw("dsposv_compact", layout, uplo, n, nrhs, ap, n, bp, ldb, xp,
+        st = cbkt.oracle().call_raw(name, layout, uplo, n, nrhs, ap, n, bp, ldb, xp,
                                     ldb, None, 0, iters, info, V, nm)
         return st, ap, xp, iters, info
     import cbk_amd
@@ -53,7 +61,7 @@ def run_dsposv(p_call, A, B, V, uplo="L", layout="C"):
     dA, dB, dX = (gpuutil.to_dev(x) for x in (ap, bp, xp))
     dI, dN = gpuutil.to_dev(iters), gpuutil.to_dev(info)
     dW = torch.empty(max(ws, 8), dtype=torch.uint8, device="cuda") if ws else None
-    st = cbk_amd.call_ext("dsposv_compact", layout, uplo, n, nrhs, dA, n, dB, ldb, dX, ldb, dW, ws,
+    st = cbk_amd.call_ext(name, layout, uplo, n, nrhs, dA, n, dB, ldb, dX, ldb, dW, ws,
                           dI, dN, V, nm)
     assert cbk_amd.synchronize() == 0, cbk_amd.last_error()
     return st, dA.cpu().numpy(), dX.cpu().numpy(), dI.cpu().numpy(), dN.cpu().numpy()
@@ -78,6 +86,8 @@ def cond1(A):
     (7, 2, 2, "U", "R", False),
     (32, 4, 32, "L", "C", False),   # V = 32: the fused kernel's matrices span two per group line
     (24, 2, 32, "U", "R", False),
+    (32, 9, 64, "L", "C", False),   # nrhs > 4: the fused kernel in blocks of 4 columns
+    (24, 24, 16, "U", "C", False),
 ])
 def test_dsposv_refines_to_fp64_accuracy(n, nrhs, V, uplo, layout, lane):
     import cbk_amd
@@ -145,3 +155,142 @@ def test_dsposv_fallback_and_info(n, V, lane):
     for m in (1, 5, nm - 1):
         tol = 4 * n * 2.0 ** -53 * cond1(A[m:m + 1])[0]
         assert np.abs(X[m] - Xo[m]).max() / np.abs(Xo[m]).max() <= tol
+
+
+def sym_indefinite_logspace(r, nm, n, logk):
+    """Q diag(+/-lambda) Q^T, log10 lambda uniform on [0, logk]: indefinite,
+    with (generically) an unpivoted LDL^T"""
+    Q = np.linalg.qr(r.standard_normal((nm, n, n)))[0]
+    lam = 10.0 ** (r.uniform(0, 1, (nm, n)) * logk)
+    lam = lam * np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    A = (Q * lam[:, None, :]) @ np.swapaxes(Q, 1, 2)
+    return 0.5 * (A + np.swapaxes(A, 1, 2))
+
+
+@pytest.mark.parametrize("n,nrhs,V,uplo,layout,lane", [
+    (24, 24, 64, "L", "C", False),  # the sysvnp benchmark shape, fused kernel
+    (32, 4, 64, "L", "C", False),
+    (16, 1, 4, "U", "R", False),
+    (24, 24, 64, "L", "C", True),   # the same through the workspace lane kernel
+    (20, 5, 8, "L", "C", False),    # an order only the lane kernel serves
+    (24, 6, 32, "U", "C", False),
+])
+def test_dssysvnp_refines_to_fp64_accuracy(n, nrhs, V, uplo, layout, lane):
+    import cbk_amd
+    r = cbkt.rng(300 + n + nrhs)
+    nm = 2 * V + 3
+    A = cbkt.gen_sym_dominant(r, nm, n)
+    B = r.standard_normal((nm, n, nrhs))
+    print(f"case dssysvnp n={n} nrhs={nrhs} V={V} {layout}{uplo} lane={lane}", flush=True)
+    prev = cbk_amd.set_policy(cbk_amd.POLICY_LANE if lane else cbk_amd.POLICY_AUTO)
+    try:
+        st, ap, xp, iters, info = run_dsposv("gpu", A, B, V, uplo, layout, "dssysvnp_compact")
+    finally:
+        cbk_amd.set_policy(prev)
+    assert st == 0
+    assert np.all(info == 0)
+    assert np.all(iters >= 0), iters
+    row = layout == "R"
+    X = cbkt.unpack(xp, nm, n, nrhs, V, row=row)
+    eta = backward_error(A, X, B)
+    assert np.all(eta <= 1e-14), eta.max()
+    Aref = np.where(np.eye(n, dtype=bool), A, np.tril(A) if uplo == "L" else np.triu(A))
+    assert np.array_equal(ap, cbkt.pack(Aref, V, row=row))
+    so, _, xo, io, no = run_dsposv("oracle", A, B, V, uplo, layout, "dssysvnp_compact")
+    assert so == 0 and np.all(io >= 0)
+    Xo = cbkt.unpack(xo, nm, n, nrhs, V, row=row)
+    tol = 4 * n * 2.0 ** -53 * cond1(A)
+    ratio = np.abs(X - Xo).max(axis=1) / np.abs(Xo).max(axis=1) / tol[:, None]
+    assert np.all(ratio <= 1.0), ratio.max()
+
+
+@pytest.mark.parametrize("n,V,lane", [(24, 64, False), (24, 64, True), (32, 8, False)])
+def test_dssysvnp_fallback_and_info(n, V, lane):
+    import cbk_amd
+    r = cbkt.rng(17 + n)
+    nm = V + 3
+    nrhs = 3
+    A = cbkt.gen_sym_dominant(r, nm, n)
+    A[[2, 6, nm - 1]] = sym_indefinite_logspace(r, 3, n, 11)  # FP32 cannot refine these
+    A[8, :, 0] = A[8, 0, :] = 0.0                             # D(1) = 0 exactly: FP64 fails too
+    A[8, 0, 0] = 0.0
+    B = r.standard_normal((nm, n, nrhs))
+    print(f"case dssysvnp fallback n={n} V={V} lane={lane}", flush=True)
+    prev = cbk_amd.set_policy(cbk_amd.POLICY_LANE if lane else cbk_amd.POLICY_AUTO)
+    try:
+        st, ap, xp, iters, info = run_dsposv("gpu", A, B, V, name="dssysvnp_compact")
+    finally:
+        cbk_amd.set_policy(prev)
+    so, apo, xo, io, no = run_dsposv("oracle", A, B, V, name="dssysvnp_compact")
+    assert st == so == 0
+    assert np.array_equal(info, no), (info, no)
+    assert info[8] == 1 and np.count_nonzero(info) == 1
+    for m in (2, 6, nm - 1, 8):
+        assert iters[m] < 0 and io[m] < 0, (m, iters[m], io[m])
+    ok = [m for m in range(nm) if m not in (2, 6, nm - 1, 8)]
+    assert np.all(iters[ok] >= 0)
+    Ag = cbkt.unpack(ap, nm, n, n, V)
+    Ao = cbkt.unpack(apo, nm, n, n, V)
+    X = cbkt.unpack(xp, nm, n, nrhs, V)
+    Xo = cbkt.unpack(xo, nm, n, nrhs, V)
+    for m in (2, 6, nm - 1):
+        # the FP64 LDL^T factor bit for bit, and the FP64 solve
+        assert np.array_equal(np.tril(Ag[m]), np.tril(Ao[m]))
+        tol = 4 * n * 2.0 ** -53 * cond1(A[m:m + 1])[0]
+        assert np.abs(X[m] - Xo[m]).max() / np.abs(Xo[m]).max() <= tol
+
+
+def iter_code_cases(n, ldl):
+    """matrices whose ITER code the construction fixes (as the oracle's own
+    test of these codes): 1, 3: -2 from B; 2: -2 from A; 4: -3 (a diagonal
+    1e-50, zero in FP32); 5: -3 with FP64 info 3; 6 (sysvnp): -3 from a D
+    whose FP32 reciprocal overflows; 0, 7: refined"""
+    r = cbkt.rng(33 + n)
+    nm = 8
+    A = cbkt.gen_spd(r, nm, n)
+    B = r.standard_normal((nm, n, 2))
+    B[1, 2, 1] = 1e39
+    A[2, 4, 1] = A[2, 1, 4] = 1e39
+    A[2, 4, 4] = 1e80
+    A[3, 1, 1] = 1e39
+    B[3, 0, 0] = -1e39
+    A[4] = np.diag(np.full(n, 2.0))
+    A[4, 0, 0] = 1e-50
+    A[5] = np.diag(np.arange(1.0, n + 1.0))
+    A[5, 2, 2] = 0.0
+    if ldl:
+        A[6] = np.eye(n) + 0.5 * (np.eye(n, k=1) + np.eye(n, k=-1))
+        A[6, 0, 0] = 1e-40
+    return A, B
+
+
+@pytest.mark.parametrize("name", ["dsposv_compact", "dssysvnp_compact"])
+@pytest.mark.parametrize("n,V,lane", [(16, 8, False), (32, 4, False), (6, 8, False),
+                                      (16, 8, True)])
+def test_mixed_iter_codes_match_oracle(name, n, V, lane):
+    import cbk_amd
+    ldl = name == "dssysvnp_compact"
+    A, B = iter_code_cases(n, ldl)
+    nm = A.shape[0]
+    print(f"case {name} iter codes n={n} V={V} lane={lane}", flush=True)
+    prev = cbk_amd.set_policy(cbk_amd.POLICY_LANE if lane else cbk_amd.POLICY_AUTO)
+    try:
+        st, ap, xp, iters, info = run_dsposv("gpu", A, B, V, name=name)
+    finally:
+        cbk_amd.set_policy(prev)
+    so, apo, xo, io, no = run_dsposv("oracle", A, B, V, name=name)
+    assert st == so == 0
+    assert np.array_equal(info, no), (info, no)
+    assert info[5] == 3
+    fixed = [1, 2, 3, 4, 5] + ([6] if ldl else [])
+    assert np.array_equal(iters[fixed], io[fixed]), (iters, io)
+    assert iters[1] == iters[2] == iters[3] == -2 and iters[4] == -3 and iters[5] == -3
+    assert np.all(iters[[0, 7]] >= 0)
+    X = cbkt.unpack(xp, nm, n, 2, V)
+    Xo = cbkt.unpack(xo, nm, n, 2, V)
+    Ag = cbkt.unpack(ap, nm, n, n, V)
+    Ao = cbkt.unpack(apo, nm, n, n, V)
+    for m in [1, 2, 3, 4] + ([6] if ldl else []):
+        # the FP64 path ran: factor bit-identical, X within the solve's rounding
+        assert np.array_equal(np.tril(Ag[m]), np.tril(Ao[m])), m
+        assert np.allclose(X[m], Xo[m], rtol=1e-12, atol=1e-12 * np.abs(Xo[m]).max()), m
