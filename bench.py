@@ -23,7 +23,8 @@ Also reported, in the same JSON line:
                residuals) at m=64, n=32, nrhs=2, 131072 problems, counting
                problems with ||A^T r||_2 / (||A||_F ||r||_2) <= 1e-14;
   configs      the other configurations (K1 dposv n=8, K2 dgels/dgeqrf/dormqr
-               16x8, K3 sysvnp and trsm 24x24), one call each;
+               16x8, K3 sysvnp and trsm 24x24, the mixed dssysvnp at K3's
+               shape, device pack/unpack of 32x32 FP64), one call each;
   uneven_gels  dgels_batch / sgels_batch over 500000 least-squares problems
                of uneven sizes (one recorded graph launch), against per-size
                dgels_compact launches on a sample of the sizes;
@@ -313,11 +314,38 @@ def configs_bench(torch):
         t = timeit(lambda: cbk_amd.call(p, "sysvnp", "C", "L", n, q, A, n, B, n, V, nm),
                    lambda: (A.copy_(A0), B.copy_(B0)))
         rec(f"K3_{p}sysvnp_n24", t, nm, es * (n * (n + 1) + 2 * n * q), f"{p}sysvnp_compact n=24 nrhs=24")
+        if p == "d" and hasattr(cbk_amd.lib(), "dssysvnp_compact"):
+            # mixed: FP32 LDL^T, FP64 refinement; A and B read, X written
+            X = torch.empty_like(B0)
+            it = torch.empty(nm, dtype=torch.int32, device="cuda")
+            inf = torch.empty_like(it)
+            t = timeit(lambda: cbk_amd.call_ext("dssysvnp_compact", "C", "L", n, q, A, n, B0, n, X, n,
+                                                None, 0, it, inf, V, nm), lambda: A.copy_(A0))
+            rec("K3_dssysvnp_n24", t, nm, 8 * (n * (n + 1) // 2 + 2 * n * q) + 8,
+                "dssysvnp_compact n=24 nrhs=24 (mixed)")
+            out["K3_dssysvnp_n24"]["refined"] = int((it >= 0).sum().item())
+            del X, it, inf
         t = timeit(lambda: cbk_amd.call(p, "trsm", "C", "L", "L", "N", "U", n, q, 1.0, A0, n, B, n, V, nm),
                    lambda: B.copy_(B0))
         rec(f"K3_{p}trsm_24x24", t, nm, es * (n * (n - 1) // 2 + 2 * n * q), f"{p}trsm_compact L L N U 24x24 nrhs=24")
         del A0, B0, A, B
     torch.cuda.empty_cache()
+
+    # dense <-> compact on the device: 131072 FP64 32 x 32 column-major
+    # matrices, V = 64 (each value read once and written once)
+    nm, n = 131072, 32
+    lib = cbk_amd.lib()
+    if hasattr(lib, "cbk_dpack_compact"):
+        D = torch.randn(nm * n * n, dtype=torch.float64, device="cuda", generator=g)
+        P = torch.empty_like(D)
+        t = timeit(lambda: cbk_amd.call_ext("cbk_dpack_compact", "C", n, n, D, "C", n, n * n, P, n,
+                                            1.0, V, nm), lambda: None)
+        rec("pack_d32", t, nm, 16 * n * n, "cbk_dpack_compact 32x32 FP64, 131072 matrices, V=64")
+        t = timeit(lambda: cbk_amd.call_ext("cbk_dunpack_compact", "C", n, n, P, n, D, "C", n, n * n,
+                                            V, nm), lambda: None)
+        rec("unpack_d32", t, nm, 16 * n * n, "cbk_dunpack_compact 32x32 FP64")
+        del D, P
+        torch.cuda.empty_cache()
     return out
 
 

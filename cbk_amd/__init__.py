@@ -98,7 +98,7 @@ def call_ext(name, *args):
     key = ("ext", name, "")
     if key not in _fns:
         kinds = abi.EXT[name]
-        _fns[key] = (abi.bind(lib(), name, kinds, "d"), kinds)
+        _fns[key] = (abi.bind(lib(), name, kinds, "s" if name.startswith("cbk_s") else "d"), kinds)
     f, kinds = _fns[key]
     if len(args) != len(kinds):
         raise TypeError(f"{name}: expected {len(kinds)} arguments, got {len(args)}")

@@ -30,6 +30,12 @@ EXT = {
     "dsposv_compact_noalloc": "cciipipipippii",
     "dssysvnp_compact": "cciipipipipzppii",
 }
+# dense <-> compact ('T' is the routine's scalar: cbk_s* float, else double)
+for _p in "ds":
+    EXT[f"cbk_{_p}pack_compact"] = "ciipcizpiTii"
+    EXT[f"cbk_{_p}unpack_compact"] = "ciipipcizii"
+    EXT[f"cbk_{_p}gepack_compact"] = "ciipipiii"
+    EXT[f"cbk_{_p}geunpack_compact"] = "ciipipiii"
 
 # per-matrix info variants: the plain argument list plus a trailing device int[nm]
 INFO_ROUTINES = ("potrf", "posv", "sytrfnp", "sysvnp", "gels")

@@ -241,6 +241,49 @@ int cbk_autotune(void);
 int cbk_get_tuning(cbk_tuning *t);
 int cbk_set_tuning(const cbk_tuning *t);
 
+/* ---- dense <-> compact ------------------------------------------------------
+ * Conversion on the device between nm dense matrices (rows x cols) and the
+ * compact layout: element (i, j) of matrix g*V+v at
+ * ap[g*gstride + (i*si + j*sj)*V + v] with (si, sj) = (1, ldap) for layout 'C',
+ * (ldap, 1) for 'R', and gstride = ldap*cols*V ('C') or ldap*rows*V ('R').
+ * Packing writes the slots of the last group beyond nm with pad_diag on the
+ * diagonal and zero elsewhere -- 1 gives the identity padding the compact
+ * routines expect, 0 the padding of a tau batch -- as the reference's test
+ * utilities pack_compact / unpack_compact (tests/test_compact_util.hpp:475-531,
+ * which fix the dense side to column-major with ld = rows; here its layout,
+ * alayout, and lda are free). Unpacking writes only the nm real matrices.
+ * Entries of ap outside the rows x cols block of each leading dimension are
+ * never read or written. Asynchronous on the cbk stream; HBM-bound (each value
+ * read once and written once).
+ *
+ * Strided dense batch: matrix idx at a + idx*stride_a (scalars), element
+ * (i, j) at i + j*lda (alayout 'C') or i*lda + j ('R').
+ * Codes (pack): -1 layout -2 rows -3 cols -4 a NULL -5 alayout -6 lda
+ *   -7 stride_a < lda * (cols or rows) with nm > 1 -8 ap NULL -9 ldap -11 V -12 nm
+ * Codes (unpack): -1 layout -2 rows -3 cols -4 ap NULL -5 ldap -6 a NULL
+ *   -7 alayout -8 lda -9 stride_a -10 V -11 nm */
+int cbk_dpack_compact(char layout, int rows, int cols, const double *a, char alayout, int lda,
+                      size_t stride_a, double *ap, int ldap, double pad_diag, int V, int nm);
+int cbk_spack_compact(char layout, int rows, int cols, const float *a, char alayout, int lda,
+                      size_t stride_a, float *ap, int ldap, float pad_diag, int V, int nm);
+int cbk_dunpack_compact(char layout, int rows, int cols, const double *ap, int ldap, double *a,
+                        char alayout, int lda, size_t stride_a, int V, int nm);
+int cbk_sunpack_compact(char layout, int rows, int cols, const float *ap, int ldap, float *a,
+                        char alayout, int lda, size_t stride_a, int V, int nm);
+
+/* MKL's mkl_?gepack_compact / mkl_?geunpack_compact argument order: a is a
+ * device array of nm device pointers, one per dense matrix, in the same
+ * layout as ap; packing pads with the identity.
+ * Codes: -1 layout -2 rows -3 cols -4 a NULL -5 lda -6 ap NULL -7 ldap -8 V -9 nm */
+int cbk_dgepack_compact(char layout, int rows, int cols, const double *const *a, int lda,
+                        double *ap, int ldap, int V, int nm);
+int cbk_sgepack_compact(char layout, int rows, int cols, const float *const *a, int lda,
+                        float *ap, int ldap, int V, int nm);
+int cbk_dgeunpack_compact(char layout, int rows, int cols, double *const *a, int lda,
+                          const double *ap, int ldap, int V, int nm);
+int cbk_sgeunpack_compact(char layout, int rows, int cols, float *const *a, int lda,
+                          const float *ap, int ldap, int V, int nm);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1034,6 +1034,193 @@ int oracle_sgels_batch(char trans, int nprob, const oracle_gels_problem *desc, f
 
 namespace {
 
+/* ---------------------------------------------------------------- */
+/* dense <-> compact: tests/test_compact_util.hpp:475-531           */
+/* ---------------------------------------------------------------- */
+
+/* pack_compact / unpack_compact of the reference's test utilities, with the
+ * dense side's layout and leading dimension free and either a strided batch
+ * or an array of matrix pointers; the library's cbk_?pack_compact family. A
+ * padded slot (idx >= nm) carries pad on its diagonal, zero elsewhere. */
+template <class T> struct DenseBatch {
+    const T *a;
+    const T *const *aptr;
+    T *w;
+    T *const *wptr;
+    size_t stride;
+    long si, sj;
+    const T *src(long idx) const { return a ? a + idx * stride : aptr[idx]; }
+    T *dst(long idx) const { return w ? w + idx * stride : wptr[idx]; }
+};
+
+inline int log2v(int V)
+{
+    for (int s = 1; s <= 6; ++s)
+        if (V == (1 << s)) return s;
+    return -1;
+}
+
+template <class T>
+void pack_loop(bool row, int rows, int cols, const DenseBatch<T> &D, T *ap, int ldap, T pad,
+               int V, int nm)
+{
+    const long ng = ((long)nm + V - 1) / V;
+    const size_t gs = group_stride(row, ldap, rows, cols, V);
+    for (long g = 0; g < ng; ++g) {
+        const LV<T> P = make_view(ap + g * gs, row, ldap, V);
+        for (long v = 0; v < V; ++v) {
+            const long idx = g * V + v;
+            for (long j = 0; j < cols; ++j)
+                for (long i = 0; i < rows; ++i)
+                    P.p[v + (i * P.si + j * P.sj) * V] =
+                        idx < nm ? D.src(idx)[i * D.si + j * D.sj] : (i == j ? pad : T(0));
+        }
+    }
+}
+
+template <class T>
+void unpack_loop(bool row, int rows, int cols, const T *ap, int ldap, const DenseBatch<T> &D,
+                 int V, int nm)
+{
+    const long ng = ((long)nm + V - 1) / V;
+    const size_t gs = group_stride(row, ldap, rows, cols, V);
+    for (long g = 0; g < ng; ++g) {
+        const LV<T> P = make_view(const_cast<T *>(ap) + g * gs, row, ldap, V);
+        for (long v = 0; v < V && g * V + v < nm; ++v)
+            for (long j = 0; j < cols; ++j)
+                for (long i = 0; i < rows; ++i)
+                    D.dst(g * V + v)[i * D.si + j * D.sj] = P.p[v + (i * P.si + j * P.sj) * V];
+    }
+}
+
+template <class T>
+int pack_strided(char layout, int rows, int cols, const T *a, char alayout, int lda,
+                 size_t stride_a, T *ap, int ldap, T pad, int V, int nm)
+{
+    const bool col = opt(layout, 'C'), row = opt(layout, 'R');
+    const bool acol = opt(alayout, 'C'), arow = opt(alayout, 'R');
+    if (!col && !row) return -1;
+    if (rows < 0) return -2;
+    if (cols < 0) return -3;
+    const bool any = nm > 0 && rows > 0 && cols > 0;
+    if (any && !a) return -4;
+    if (!acol && !arow) return -5;
+    if (lda < max1(arow ? cols : rows)) return -6;
+    if (nm > 1 && stride_a < (size_t)lda * (arow ? rows : cols)) return -7;
+    if (any && !ap) return -8;
+    if (ldap < max1(row ? cols : rows)) return -9;
+    if (log2v(V) < 0) return -11;
+    if (nm < 0) return -12;
+    if (!any) return 0;
+    DenseBatch<T> D{a, nullptr, nullptr, nullptr, stride_a, arow ? lda : 1, arow ? 1 : lda};
+    pack_loop(row, rows, cols, D, ap, ldap, pad, V, nm);
+    return 0;
+}
+
+template <class T>
+int unpack_strided(char layout, int rows, int cols, const T *ap, int ldap, T *a, char alayout,
+                   int lda, size_t stride_a, int V, int nm)
+{
+    const bool col = opt(layout, 'C'), row = opt(layout, 'R');
+    const bool acol = opt(alayout, 'C'), arow = opt(alayout, 'R');
+    if (!col && !row) return -1;
+    if (rows < 0) return -2;
+    if (cols < 0) return -3;
+    const bool any = nm > 0 && rows > 0 && cols > 0;
+    if (any && !ap) return -4;
+    if (ldap < max1(row ? cols : rows)) return -5;
+    if (any && !a) return -6;
+    if (!acol && !arow) return -7;
+    if (lda < max1(arow ? cols : rows)) return -8;
+    if (nm > 1 && stride_a < (size_t)lda * (arow ? rows : cols)) return -9;
+    if (log2v(V) < 0) return -10;
+    if (nm < 0) return -11;
+    if (!any) return 0;
+    DenseBatch<T> D{nullptr, nullptr, a, nullptr, stride_a, arow ? lda : 1, arow ? 1 : lda};
+    unpack_loop(row, rows, cols, ap, ldap, D, V, nm);
+    return 0;
+}
+
+template <class T>
+int gepack(char layout, int rows, int cols, const T *const *a, int lda, T *ap, int ldap, int V,
+           int nm)
+{
+    const bool col = opt(layout, 'C'), row = opt(layout, 'R');
+    if (!col && !row) return -1;
+    if (rows < 0) return -2;
+    if (cols < 0) return -3;
+    const bool any = nm > 0 && rows > 0 && cols > 0;
+    if (any && !a) return -4;
+    if (lda < max1(row ? cols : rows)) return -5;
+    if (any && !ap) return -6;
+    if (ldap < max1(row ? cols : rows)) return -7;
+    if (log2v(V) < 0) return -8;
+    if (nm < 0) return -9;
+    if (!any) return 0;
+    DenseBatch<T> D{nullptr, a, nullptr, nullptr, 0, row ? lda : 1, row ? 1 : lda};
+    pack_loop(row, rows, cols, D, ap, ldap, T(1), V, nm);
+    return 0;
+}
+
+template <class T>
+int geunpack(char layout, int rows, int cols, T *const *a, int lda, const T *ap, int ldap, int V,
+             int nm)
+{
+    const bool col = opt(layout, 'C'), row = opt(layout, 'R');
+    if (!col && !row) return -1;
+    if (rows < 0) return -2;
+    if (cols < 0) return -3;
+    const bool any = nm > 0 && rows > 0 && cols > 0;
+    if (any && !a) return -4;
+    if (lda < max1(row ? cols : rows)) return -5;
+    if (any && !ap) return -6;
+    if (ldap < max1(row ? cols : rows)) return -7;
+    if (log2v(V) < 0) return -8;
+    if (nm < 0) return -9;
+    if (!any) return 0;
+    DenseBatch<T> D{nullptr, nullptr, nullptr, a, 0, row ? lda : 1, row ? 1 : lda};
+    unpack_loop(row, rows, cols, ap, ldap, D, V, nm);
+    return 0;
+}
+
+} /* namespace */
+
+extern "C" {
+
+#define ORACLE_PACK(T, p)                                                                     \
+    int oracle_cbk_##p##pack_compact(char layout, int rows, int cols, const T *a,             \
+                                     char alayout, int lda, size_t stride_a, T *ap, int ldap, \
+                                     T pad_diag, int V, int nm)                               \
+    {                                                                                         \
+        return pack_strided<T>(layout, rows, cols, a, alayout, lda, stride_a, ap, ldap,       \
+                               pad_diag, V, nm);                                              \
+    }                                                                                         \
+    int oracle_cbk_##p##unpack_compact(char layout, int rows, int cols, const T *ap,          \
+                                       int ldap, T *a, char alayout, int lda,                 \
+                                       size_t stride_a, int V, int nm)                        \
+    {                                                                                         \
+        return unpack_strided<T>(layout, rows, cols, ap, ldap, a, alayout, lda, stride_a, V,  \
+                                 nm);                                                         \
+    }                                                                                         \
+    int oracle_cbk_##p##gepack_compact(char layout, int rows, int cols, const T *const *a,    \
+                                       int lda, T *ap, int ldap, int V, int nm)               \
+    {                                                                                         \
+        return gepack<T>(layout, rows, cols, a, lda, ap, ldap, V, nm);                        \
+    }                                                                                         \
+    int oracle_cbk_##p##geunpack_compact(char layout, int rows, int cols, T *const *a,        \
+                                         int lda, const T *ap, int ldap, int V, int nm)       \
+    {                                                                                         \
+        return geunpack<T>(layout, rows, cols, a, lda, ap, ldap, V, nm);                      \
+    }
+
+ORACLE_PACK(double, d)
+ORACLE_PACK(float, s)
+#undef ORACLE_PACK
+
+} /* extern "C" */
+
+namespace {
+
 /* LAPACK 3.12 dsposv.f, per matrix, on the compact layout; LDL: the same
  * algorithm with the unpivoted LDL^T of sysvnp in both precisions. The inner
  * single precision factor and solves are this file's restatement of the

@@ -84,6 +84,30 @@ int oracle_dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, i
                           const double *bp, int ldbp, double *xp, int ldxp, void *work,
                           size_t lwork, int *iter, int *info, int V, int nm);
 
+/* Dense <-> compact conversion (the reference's test utilities pack_compact /
+ * unpack_compact, tests/test_compact_util.hpp:475-531, with the dense layout,
+ * lda and batch stride free); host pointers, the library's cbk_?pack_compact
+ * family and arguments. The gepack forms take an array of nm matrix pointers
+ * in the compact layout's own layout and pad with the identity. */
+int oracle_cbk_dpack_compact(char layout, int rows, int cols, const double *a, char alayout,
+                             int lda, size_t stride_a, double *ap, int ldap, double pad_diag,
+                             int V, int nm);
+int oracle_cbk_spack_compact(char layout, int rows, int cols, const float *a, char alayout,
+                             int lda, size_t stride_a, float *ap, int ldap, float pad_diag,
+                             int V, int nm);
+int oracle_cbk_dunpack_compact(char layout, int rows, int cols, const double *ap, int ldap,
+                               double *a, char alayout, int lda, size_t stride_a, int V, int nm);
+int oracle_cbk_sunpack_compact(char layout, int rows, int cols, const float *ap, int ldap,
+                               float *a, char alayout, int lda, size_t stride_a, int V, int nm);
+int oracle_cbk_dgepack_compact(char layout, int rows, int cols, const double *const *a, int lda,
+                               double *ap, int ldap, int V, int nm);
+int oracle_cbk_sgepack_compact(char layout, int rows, int cols, const float *const *a, int lda,
+                               float *ap, int ldap, int V, int nm);
+int oracle_cbk_dgeunpack_compact(char layout, int rows, int cols, double *const *a, int lda,
+                                 const double *ap, int ldap, int V, int nm);
+int oracle_cbk_sgeunpack_compact(char layout, int rows, int cols, float *const *a, int lda,
+                                 const float *ap, int ldap, int V, int nm);
+
 /* dsposv's algorithm with the unpivoted LDL^T of sysvnp in both precisions
  * (no reference counterpart, no LAPACK routine): the single factor also fails
  * on a non-finite D; the fallback is the double dsysvnp, after which ap holds

@@ -170,7 +170,8 @@ class CLib:
         key = ("ext", name)
         if key not in self._fns:
             kinds = abi.EXT[name]
-            self._fns[key] = (abi.bind(self.lib, self.prefix + name, kinds, "d"), kinds)
+            scalar = "s" if name.startswith("cbk_s") else "d"
+            self._fns[key] = (abi.bind(self.lib, self.prefix + name, kinds, scalar), kinds)
         f, kinds = self._fns[key]
         assert len(args) == len(kinds), (name, len(args), len(kinds))
         cargs = []
