@@ -25,9 +25,11 @@ Also reported, in the same JSON line:
   configs      the other configurations (K1 dposv n=8, K2 dgels/dgeqrf/dormqr
                16x8, K3 sysvnp and trsm 24x24, the mixed dssysvnp at K3's
                shape, device pack/unpack of 32x32 FP64), one call each;
-  uneven_gels  dgels_batch / sgels_batch over 500000 least-squares problems
-               of uneven sizes (one recorded graph launch), against per-size
-               dgels_compact launches on a sample of the sizes;
+  uneven_gels  dgels_batch / sgels_batch / dsgels_batch (FP32 batch QR +
+               FP64 refinement + FP64 fallback) over 500000 least-squares
+               problems of uneven sizes (one recorded graph launch each),
+               against one dgels_compact launch per distinct size (all 16104
+               of them), with the LAPACK-test LS backward error of a sample;
   roofline     of the dominant kernel (algorithmic bytes per launch / its
                average duration from HIP events on the launch stream, against
                MI355X's 8 TB/s; traffic from the rocprofv3 counters recorded in
@@ -350,7 +352,6 @@ def configs_bench(torch):
 
 
 K5_NPROB = 500000
-PER_SIZE_SAMPLE = 256
 
 
 def uneven_inputs(nprob, seed=2027):
@@ -373,6 +374,27 @@ def uneven_inputs(nprob, seed=2027):
     return desc, A, B
 
 
+def ls_backward_error(desc, Ah, Bh, Xpool, sample):
+    """the largest least-squares backward-error ratio of LAPACK's own tests
+    (dqrt17) over the sampled problems and their columns:
+    ||A^T r||_2 / (||A||_F (||A||_F ||x||_2 + ||b||_2)), r = b - A x; an FP64
+    Householder solve gives eps-level values"""
+    import numpy as np
+    worst = 0.0
+    for k in sample:
+        mk, nk, qk = int(desc["m"][k]), int(desc["n"][k]), int(desc["nrhs"][k])
+        a0, b0 = int(desc["a_off"][k]), int(desc["b_off"][k])
+        A = Ah[a0:a0 + mk * nk].reshape(nk, mk).T
+        B = Bh[b0:b0 + mk * qk].reshape(qk, mk).T
+        X = Xpool[b0:b0 + mk * qk].reshape(qk, mk).T[:nk]
+        R = B - A @ X
+        an = np.linalg.norm(A)
+        eta = np.linalg.norm(A.T @ R, axis=0) / (an * (an * np.linalg.norm(X, axis=0) +
+                                                       np.linalg.norm(B, axis=0)))
+        worst = max(worst, float(eta.max()))
+    return worst
+
+
 def uneven_bench(torch, K, W, nprob=K5_NPROB):
     """dgels_batch / sgels_batch over the uneven batch (one graph launch per
     step, inputs re-copied outside the timed region), against the same problems
@@ -384,6 +406,7 @@ def uneven_bench(torch, K, W, nprob=K5_NPROB):
     log(f"generating the uneven batch ({nprob} problems) ...")
     desc, Ah, Bh = uneven_inputs(nprob)
     m, n, q = desc["m"], desc["n"], desc["nrhs"]
+    sample = np.random.default_rng(7).choice(nprob, 4000, replace=False)
     nbytes = {"D": 8, "S": 4}
     out = {"workload": f"gels, {nprob} problems, m~U[4,64], n~U[2,m], nrhs~U[1,8], U(-1,1), "
                        "column-major per problem",
@@ -439,10 +462,51 @@ def uneven_bench(torch, K, W, nprob=K5_NPROB):
                 Xo = batchcase.get_b(sbp, sdesc[j])[:nk]
                 worst = max(worst, float(np.abs(Xg - Xo).max() / max(np.abs(Xo).max(), 1e-300)))
             res["sample_vs_oracle"] = {"problems": ns, "max_rel_diff_x": worst}
+        if prec == "D":
+            res["sample_ls_backward_error"] = ls_backward_error(desc, Ah, Bh, B.cpu().numpy(), sample)
         out["fp64" if prec == "D" else "fp32"] = res
         log(f"uneven gels {prec}: {t * 1e3:.2f} ms/step, {nprob / t:.3e} problems/s, "
             f"{traffic / t / 1e9:.0f} GB/s, classes {plan.classes()}")
         del A0, B0, A, B, work, plan
+        torch.cuda.empty_cache()
+
+    # the mixed form: FP32 batch QR, FP64 refinement, FP64 fallback, one graph
+    if hasattr(cbk_amd.lib(), "dsgels_batch"):
+        A0 = torch.from_numpy(Ah).cuda()
+        B0 = torch.from_numpy(Bh).cuda()
+        A = A0.clone()
+        X = torch.empty_like(B0)
+        plan = cbk_amd.GelsBatchPlan("M", "N", desc)
+        work = torch.empty(max(plan.worksize, 1), dtype=torch.uint8, device="cuda")
+        it = torch.empty(nprob, dtype=torch.int32, device="cuda")
+        info = torch.empty(nprob, dtype=torch.int32, device="cuda")
+        ts = []
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        stream = torch.cuda.current_stream()
+        for s_ in range(W + K):
+            A.copy_(A0)
+            e0.record(stream)
+            assert plan.run_mixed(A, B0, X, it, info, work) == 0, cbk_amd.last_error()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if s_ >= W:
+                ts.append(e0.elapsed_time(e1) / 1e3)
+        t = sum(ts) / len(ts)
+        itc = it.cpu().numpy()
+        vals, cnts = np.unique(itc, return_counts=True)
+        # A and B read, X written (refined problems leave A untouched)
+        traffic = 8 * int((m * n + 2 * m * q).sum())
+        res = {"ms": t * 1e3, "problems_per_s": nprob / t, "hbm_gbs": traffic / t / 1e9,
+               "bytes": traffic, "iter_histogram": {str(int(v)): int(c) for v, c in zip(vals, cnts)},
+               "fp64_fallback": int((itc < 0).sum()), "nonzero_info": int((info != 0).sum().item()),
+               "workspace_bytes": plan.worksize,
+               "sample_ls_backward_error": ls_backward_error(desc, Ah, Bh, X.cpu().numpy(), sample)}
+        out["mixed"] = res
+        out["mixed_over_fp64"] = out["fp64"]["ms"] / res["ms"]
+        log(f"uneven dsgels_batch: {t * 1e3:.2f} ms/step, {nprob / t:.3e} problems/s, "
+            f"FP64 fallbacks {res['fp64_fallback']}, iters {res['iter_histogram']}, "
+            f"LS backward error (sample) {res['sample_ls_backward_error']:.2e}")
+        del A0, B0, A, X, work, plan
         torch.cuda.empty_cache()
 
     # per-size compact launches, FP64
@@ -455,9 +519,9 @@ def uneven_bench(torch, K, W, nprob=K5_NPROB):
     ends = np.concatenate([bounds, [nprob]])
     V = 64
     ntotal = len(starts)
-    # a random sample of the sizes keeps this leg to seconds; the rate is the
-    # sampled problems over the sampled launches' time
-    pick = np.sort(np.random.default_rng(11).choice(ntotal, min(ntotal, PER_SIZE_SAMPLE), replace=False))
+    # every distinct size, one dgels_compact launch each (the groups padded to
+    # whole V = 64 groups with identity matrices, as the compact API needs)
+    pick = np.arange(ntotal)
     groups, apos, bpos = [], 0, 0
     for s0, s1 in zip(starts[pick], ends[pick]):
         ids = order[s0:s1]
@@ -500,11 +564,12 @@ def uneven_bench(torch, K, W, nprob=K5_NPROB):
     torch.cuda.synchronize()
     tp = e0.elapsed_time(e1) / 1e3
     nsamp = sum(g[3] for g in groups)
-    out["per_size_fp64"] = {"ms_sampled": tp * 1e3, "problems_per_s": nsamp / tp,
-                            "launches_total": ntotal, "launches_sampled": len(groups),
-                            "problems_sampled": nsamp}
+    out["per_size_fp64"] = {"ms": tp * 1e3, "problems_per_s": nsamp / tp,
+                            "launches": len(groups), "problems": nsamp}
     out["batch_over_per_size_fp64"] = out["fp64"]["problems_per_s"] / out["per_size_fp64"]["problems_per_s"]
-    log(f"per-size dgels_compact: {len(groups)} of {ntotal} sizes, {nsamp} problems, {tp * 1e3:.1f} ms")
+    if "mixed" in out:
+        out["mixed_over_per_size_fp64"] = out["mixed"]["problems_per_s"] / out["per_size_fp64"]["problems_per_s"]
+    log(f"per-size dgels_compact: all {ntotal} sizes, {nsamp} problems, {tp * 1e3:.1f} ms")
     del dA, dB, dT
     torch.cuda.empty_cache()
     return out

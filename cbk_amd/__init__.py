@@ -191,6 +191,8 @@ def _bind_batch():
         f = getattr(L, name)
         f.argtypes = [P, P, P, P, P, ctypes.c_size_t]
         f.restype = ctypes.c_int
+    L.dsgels_batch.argtypes = [P, P, P, P, P, P, P, ctypes.c_size_t]
+    L.dsgels_batch.restype = ctypes.c_int
     L.cbk_batch_plan_destroy.argtypes = [P]
     L.cbk_batch_plan_destroy.restype = ctypes.c_int
     L.cbk_autotune.argtypes = []
@@ -240,6 +242,13 @@ class GelsBatchPlan:
         if lwork is None:
             lwork = work.numel() * work.element_size() if hasattr(work, "numel") else 0
         return f(self._h, _ptr(ap), _ptr(bp), _ptr(info), _ptr(work), lwork)
+
+    def run_mixed(self, ap, bp, xp, iters, info, work, lwork=None):
+        """dsgels_batch (a plan made with prec 'M') on device tensors"""
+        if lwork is None:
+            lwork = work.numel() * work.element_size() if hasattr(work, "numel") else 0
+        return _bind_batch().dsgels_batch(self._h, _ptr(ap), _ptr(bp), _ptr(xp), _ptr(iters),
+                                          _ptr(info), _ptr(work), lwork)
 
     def close(self):
         if self._h:

@@ -37,6 +37,7 @@ hipError_t launch_qr_wave_masked(int p, int q, int nrhs, Operand<double> A, Oper
     args.x_rs = (X.row ? X.ld : 1) * V;
     args.x_cs = (X.row ? 1 : X.ld) * V;
     args.x_gs = (long long)X.gstride;
+    args.dcount = nullptr;
     /* persistent: one wavefront per 64-entry chunk of the mask */
     const unsigned blocks = (unsigned)(((nm + 63) / 64 + 3) / 4);
     if (p <= 32)

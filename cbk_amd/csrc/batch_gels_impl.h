@@ -71,13 +71,14 @@ __global__ __launch_bounds__(256) void gels_team_kernel(BatchArgs<T> p)
     const int c = threadIdx.x & (TS - 1);
     const int sub = (threadIdx.x >> 4) & 3; /* team within the wavefront */
     const int lane = threadIdx.x & 63;
+    const unsigned count = batch_count(p);
     for (;;) {
         unsigned base = 0;
         if (lane == 0) base = atomicAdd(p.counter, 4u);
         base = __builtin_amdgcn_readfirstlane(base);
-        if (base >= (unsigned)p.count) return; /* the whole wavefront */
+        if (base >= count) return; /* the whole wavefront */
         const int k = (int)base + sub;
-        const bool live = k < p.count;
+        const bool live = (unsigned)k < count;
         const BatchProblem d = p.desc[live ? k : (int)base];
         const int m = d.m, n = live ? d.n : 0, w = live ? d.n + d.nrhs : 0;
         const bool isA = c < n, isB = c >= n && c < w;
@@ -217,6 +218,7 @@ gels_wave_kernel(BatchArgs<T> p)
     unsigned chunk = blockIdx.x * 4u + (unsigned)wave_in_block();
     unsigned long long pend = 0;
     bool first_chunk = true;
+    const unsigned count = batch_count(p);
     for (int round = 0;; ++round) {
         unsigned kq = 0;
         if (p.counter) {
@@ -241,7 +243,7 @@ gels_wave_kernel(BatchArgs<T> p)
             if (round > 0) return;
             kq = xcd_block(blockIdx.x) * 4u + (unsigned)wave_in_block();
         }
-        if (kq >= (unsigned)p.count) return;
+        if (kq >= count) return;
         const BatchProblem d = problem_at(p, kq);
         const int m = d.m, n = d.n, w = d.n + d.nrhs;
         const bool isA = c < n, isB = c >= n && c < w;
@@ -385,13 +387,14 @@ gels_wave_pair_kernel(BatchArgs<T> p)
         return op(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 32));
     };
     auto imax = [](int a, int b) { return a > b ? a : b; };
+    const unsigned count = batch_count(p);
     for (;;) {
         unsigned k0q = 0;
         if (lane == 0) k0q = atomicAdd(p.counter, 2u);
         k0q = __builtin_amdgcn_readfirstlane(k0q);
-        if (k0q >= (unsigned)p.count) return;
+        if (k0q >= count) return;
         const unsigned kq = k0q + (unsigned)hf;
-        const bool real = kq < (unsigned)p.count;
+        const bool real = kq < count;
         const BatchProblem d = p.desc[real ? kq : k0q];
         const int m = real ? d.m : 0, n = real ? d.n : 0, w = real ? d.n + d.nrhs : 0;
         const int nmax = both(n, imax), mmax = both(m, imax);
@@ -513,11 +516,12 @@ __global__ __launch_bounds__(256) void gels_wave2_kernel(BatchArgs<T> p)
 {
     constexpr int MMAX = 64;
     const int c = threadIdx.x & 63;
+    const unsigned count = batch_count(p);
     for (;;) {
         unsigned kq = 0;
         if (c == 0) kq = atomicAdd(p.counter, 1u);
         kq = __builtin_amdgcn_readfirstlane(kq);
-        if (kq >= (unsigned)p.count) return;
+        if (kq >= count) return;
         const BatchProblem d = p.desc[kq];
         const int m = d.m, n = d.n;
         const bool isA = c < n, isB = c < d.nrhs;
@@ -693,6 +697,7 @@ hipError_t launch_qr_wave(int p, int q, int nrhs, bool transposed, Operand<T> A,
     args.x = nullptr;
     args.x_rs = args.x_cs = 0;
     args.x_gs = 0;
+    args.dcount = nullptr;
     const unsigned blocks = team::xcd_grid((unsigned)((args.count + 3) / 4));
     if (p <= 32)
         hipLaunchKernelGGL((batch::gels_wave_kernel<T, 32>), dim3(blocks), dim3(256), 0,

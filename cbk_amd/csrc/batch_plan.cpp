@@ -1,13 +1,22 @@
 /* batch_plan.cpp -- uneven batches: the plan (size classes, queue order,
- * workspace layout, the recorded hipGraph), the dgels_batch / sgels_batch
- * entry points and the tuner that picks each class's kernel
+ * workspace layout, the recorded hipGraph), the dgels_batch / sgels_batch /
+ * dsgels_batch entry points and the tuner that picks each class's kernel
  * (include/cbk_ext.h declares the ABI; no reference counterpart).
  *
- * Workspace layout (device, caller-owned):
+ * Workspace layout of a plain plan (device, caller-owned):
  *   [0, 256)                 one 32-bit work counter per class
  *   [256, 256 + 48 nprob)    the problem table, grouped by class, each class
  *                            ordered by estimated work, largest first
  *   then, 256-aligned        tau scratch of the general kernel's problems
+ *
+ * A mixed plan (dsgels_batch) keeps, each 256-aligned after the counters
+ * (FP32 class queues, FP64 class queues, FP64 list fills): the FP32 table of
+ * the problems the FP32 path serves, the FP64 fallback lists (one segment
+ * per kernel, sized for every problem of that kernel, the problems without
+ * an FP32 path placed at their starts at plan time), the refinement table
+ * grouped by shape bucket, the list of the problems without an FP32 path,
+ * the counters' initial values, the FP32 info scratch, the FP64 tau scratch
+ * and the FP32 copies of A and B (at the caller's offsets).
  */
 
 #include "cbk.h"
@@ -92,6 +101,29 @@ int class_of_kernel(int k) { return k >= kBatchClasses ? k - 4 : k; }
 
 int problems_per_block(int k) { return k <= 2 ? 16 : k <= 5 ? 4 : k == kBatchGeneral ? 256 : 8; }
 
+/* a problem the mixed path refines: trans 'N', 1 <= n <= m <= 64,
+ * 1 <= nrhs <= 8 */
+bool mixed_eligible(const cbk_gels_problem &p, bool tran)
+{
+    return !tran && p.n >= 1 && p.n <= p.m && p.m <= 64 && p.nrhs >= 1 && p.nrhs <= 8;
+}
+
+constexpr int kMixedCounters = 3 * kBatchKernels; /* FP32 queues, FP64 queues, list fills */
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+struct MixedLayout {
+    std::vector<BatchProblem> table32, fbtable, inel;
+    std::vector<cbk::MixedProblem> reftable;
+    int begin32[kBatchKernels + 1] = {};
+    int fb_begin[kBatchKernels] = {}, fb_cap[kBatchKernels] = {};
+    unsigned init[kMixedCounters] = {};
+    int ref_begin[cbk::kRefineBuckets + 1] = {};
+    long long ext_a = 0, ext_b = 0, tau64 = 0;
+    size_t off32 = 0, offfb = 0, offref = 0, offinel = 0, offinit = 0, offinfo = 0, offtau = 0,
+           offa = 0, offb = 0;
+};
+
 } /* namespace */
 
 struct cbk_batch_plan {
@@ -111,6 +143,10 @@ struct cbk_batch_plan {
     void *key[4] = {nullptr, nullptr, nullptr, nullptr};
     hipStream_t cap = nullptr, side[kBatchKernels] = {};
     hipEvent_t fork = nullptr, join[kBatchKernels] = {};
+    /* mixed plans (prec 'M') */
+    MixedLayout *mx = nullptr;
+    void *mkey[6] = {};
+    hipEvent_t fork2 = nullptr, join2[kBatchKernels] = {};
 
     ~cbk_batch_plan()
     {
@@ -122,6 +158,10 @@ struct cbk_batch_plan {
         }
         if (cap) (void)hipStreamDestroy(cap);
         if (fork) (void)hipEventDestroy(fork);
+        for (int c = 0; c < kBatchKernels; ++c)
+            if (join2[c]) (void)hipEventDestroy(join2[c]);
+        if (fork2) (void)hipEventDestroy(fork2);
+        delete mx;
     }
 };
 
@@ -132,25 +172,29 @@ hipError_t ensure_streams(cbk_batch_plan *pl)
     if (pl->cap) return hipSuccess;
     hipError_t e = hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->fork2, hipEventDisableTiming);
     for (int c = 0; c < kBatchKernels && e == hipSuccess; ++c) {
         e = hipStreamCreateWithFlags(&pl->side[c], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->join[c], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->join2[c], hipEventDisableTiming);
     }
     return e;
 }
 
+/* the arguments of one class launch over count problems at desc */
 template <class T>
-BatchArgs<T> class_args(const cbk_batch_plan *pl, int c, T *ap, T *bp, int *info, char *work)
+BatchArgs<T> list_args(T *ap, T *bp, int *info, const BatchProblem *desc, int count,
+                       unsigned *counter, T *tau, bool tran, const unsigned *dcount)
 {
     BatchArgs<T> a;
     a.a = ap;
     a.b = bp;
     a.info = info;
-    a.desc = reinterpret_cast<const BatchProblem *>(work + pl->table_off) + pl->begin[c];
-    a.counter = reinterpret_cast<unsigned *>(work) + c;
-    a.count = pl->begin[c + 1] - pl->begin[c];
-    a.tau = reinterpret_cast<T *>(work + pl->tau_off);
-    a.tran = pl->tran;
+    a.desc = desc;
+    a.counter = counter;
+    a.count = count;
+    a.tau = tau;
+    a.tran = tran;
     a.a_rs = a.b_rs = 1;
     a.ninfo = 0x7fffffff;
     a.cm = a.cn = a.cnrhs = a.cV = a.a_cs = a.b_cs = 0;
@@ -160,15 +204,25 @@ BatchArgs<T> class_args(const cbk_batch_plan *pl, int c, T *ap, T *bp, int *info
     a.x = nullptr;
     a.x_rs = a.x_cs = 0;
     a.x_gs = 0;
+    a.dcount = dcount;
     return a;
+}
+
+template <class T>
+BatchArgs<T> class_args(const cbk_batch_plan *pl, int c, T *ap, T *bp, int *info, char *work)
+{
+    return list_args<T>(ap, bp, info,
+                        reinterpret_cast<const BatchProblem *>(work + pl->table_off) + pl->begin[c],
+                        pl->begin[c + 1] - pl->begin[c], reinterpret_cast<unsigned *>(work) + c,
+                        reinterpret_cast<T *>(work + pl->tau_off), pl->tran, nullptr);
 }
 
 /* blocks of class c's persistent grid: enough resident blocks to fill the
  * device (the tuned count per compute unit, else the kernel's occupancy), no
  * more than the class has problems for */
-template <class T> hipError_t class_blocks(const cbk_batch_plan *pl, int c, unsigned *blocks)
+template <class T>
+hipError_t class_blocks_n(const cbk_batch_plan *pl, int c, long count, unsigned *blocks)
 {
-    const long count = pl->begin[c + 1] - pl->begin[c];
     const long need = (count + problems_per_block(c) - 1) / problems_per_block(c);
     if (c == kBatchGeneral) {
         *blocks = (unsigned)need;
@@ -186,13 +240,23 @@ template <class T> hipError_t class_blocks(const cbk_batch_plan *pl, int c, unsi
     return hipSuccess;
 }
 
+template <class T> hipError_t class_blocks(const cbk_batch_plan *pl, int c, unsigned *blocks)
+{
+    return class_blocks_n<T>(pl, c, pl->begin[c + 1] - pl->begin[c], blocks);
+}
+
+template <class T>
+hipError_t launch_args(int c, const BatchArgs<T> &args, unsigned blocks, hipStream_t s)
+{
+    if (c == kBatchGeneral) return cbk::launch_gels_batch_lane<T>(args, s);
+    return cbk::launch_gels_batch<T>(c, args, blocks, s);
+}
+
 template <class T>
 hipError_t launch_class(const cbk_batch_plan *pl, int c, unsigned blocks, T *ap, T *bp,
                         int *info, char *work, hipStream_t s)
 {
-    const BatchArgs<T> args = class_args<T>(pl, c, ap, bp, info, work);
-    if (c == kBatchGeneral) return cbk::launch_gels_batch_lane<T>(args, s);
-    return cbk::launch_gels_batch<T>(c, args, blocks, s);
+    return launch_args<T>(c, class_args<T>(pl, c, ap, bp, info, work), blocks, s);
 }
 
 /* counters cleared by a kernel (no memset node), then every non-empty class
@@ -262,6 +326,236 @@ int gels_batch(cbk_batch_plan *pl, char prec, T *ap, T *bp, int *info, void *wor
     if (!pl->exec || pl->key[0] != (void *)ap || pl->key[1] != (void *)bp ||
         pl->key[2] != (void *)info || pl->key[3] != work) {
         e = record_graph<T>(pl, ap, bp, info, w);
+        if (e != hipSuccess) return cbk::launch_status(e);
+    }
+    return cbk::launch_status(hipGraphLaunch(pl->exec, st));
+}
+
+/* ---- the mixed-precision batch ---------------------------------------------- */
+
+/* sort the problems into the mixed plan's tables (host only) */
+void build_mixed(cbk_batch_plan *pl, const cbk_gels_problem *desc)
+{
+    MixedLayout &L = *pl->mx;
+    const int np = pl->nprob;
+    std::vector<int> k64(np), el(np);
+    for (int k = 0; k < np; ++k) {
+        k64[k] = kernel_of(classify(desc[k], pl->tran, pl->tune), desc[k]);
+        el[k] = mixed_eligible(desc[k], pl->tran);
+        L.fb_cap[k64[k]]++;
+    }
+    for (int c = 1; c < kBatchKernels; ++c)
+        L.fb_begin[c] = L.fb_begin[c - 1] + L.fb_cap[c - 1];
+    L.fbtable.assign(np, BatchProblem{0, 0, 0, 1, 1, 0, 0, 0, 0});
+    std::vector<int> order;
+    for (int k = 0; k < np; ++k) {
+        const cbk_gels_problem &p = desc[k];
+        BatchProblem b{p.m, p.n, p.nrhs, p.lda, p.ldb, k, p.a_off, p.b_off, 0};
+        if (!el[k]) {
+            /* no FP32 path: the FP64 gels runs on it directly */
+            if (k64[k] == kBatchGeneral) {
+                b.tau_off = L.tau64;
+                L.tau64 += std::min(p.m, p.n);
+            }
+            L.fbtable[L.fb_begin[k64[k]] + L.init[2 * kBatchKernels + k64[k]]++] = b;
+            L.inel.push_back(b);
+            continue;
+        }
+        order.push_back(k);
+        L.ext_a = std::max(L.ext_a, p.a_off + (long long)p.lda * (p.n - 1) + p.m);
+        L.ext_b = std::max(L.ext_b, p.b_off + (long long)p.ldb * (p.nrhs - 1) + p.m);
+    }
+    /* FP32 classes as the FP64 ones (the classes do not depend on precision),
+     * each ordered longest first */
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (k64[x] != k64[y]) return k64[x] < k64[y];
+        return work_of(desc[x]) > work_of(desc[y]);
+    });
+    for (int k : order) {
+        const cbk_gels_problem &p = desc[k];
+        L.table32.push_back(BatchProblem{p.m, p.n, p.nrhs, p.lda, p.ldb, k, p.a_off, p.b_off, 0});
+        L.begin32[k64[k] + 1]++;
+    }
+    for (int c = 0; c < kBatchKernels; ++c)
+        L.begin32[c + 1] += L.begin32[c];
+    std::vector<int> bucket(np, 0);
+    for (int k : order)
+        bucket[k] = cbk::mixed_refine_bucket(desc[k].m, desc[k].n, desc[k].nrhs);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (bucket[x] != bucket[y]) return bucket[x] < bucket[y];
+        return work_of(desc[x]) > work_of(desc[y]);
+    });
+    for (int k : order) {
+        const cbk_gels_problem &p = desc[k];
+        L.reftable.push_back(
+            cbk::MixedProblem{p.m, p.n, p.nrhs, p.lda, p.ldb, k, p.a_off, p.b_off, k64[k], 0});
+        L.ref_begin[bucket[k] + 1]++;
+    }
+    for (int b = 0; b < cbk::kRefineBuckets; ++b)
+        L.ref_begin[b + 1] += L.ref_begin[b];
+    size_t off = kCounterBytes;
+    L.off32 = off;
+    off = align256(off + L.table32.size() * sizeof(BatchProblem));
+    L.offfb = off;
+    off = align256(off + L.fbtable.size() * sizeof(BatchProblem));
+    L.offref = off;
+    off = align256(off + L.reftable.size() * sizeof(cbk::MixedProblem));
+    L.offinel = off;
+    off = align256(off + L.inel.size() * sizeof(BatchProblem));
+    L.offinit = off;
+    off = align256(off + sizeof(L.init));
+    L.offinfo = off;
+    off = align256(off + (size_t)np * sizeof(int));
+    L.offtau = off;
+    off = align256(off + (size_t)L.tau64 * sizeof(double));
+    L.offa = off;
+    off = align256(off + (size_t)L.ext_a * sizeof(float));
+    L.offb = off;
+    off = align256(off + (size_t)L.ext_b * sizeof(float));
+    pl->bytes = off;
+}
+
+template <class E>
+hipError_t upload(char *w, size_t off, const std::vector<E> &v, hipStream_t s)
+{
+    if (v.empty()) return hipSuccess;
+    return hipMemcpyAsync(w + off, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice, s);
+}
+
+/* counters from their initial values, the conversion and copies, the FP32
+ * classes on their branches, the refinement buckets, then the FP64 classes
+ * over the fallback lists on their branches */
+hipError_t record_graph_mixed(cbk_batch_plan *pl, double *ap, const double *bp, double *xp,
+                              int *iter, int *info, char *work)
+{
+    MixedLayout &L = *pl->mx;
+    hipError_t e = ensure_streams(pl);
+    if (e != hipSuccess) return e;
+    unsigned blocks32[kBatchKernels] = {}, blocks64[kBatchKernels] = {};
+    for (int c = 0; c < kBatchKernels && e == hipSuccess; ++c) {
+        if (L.begin32[c + 1] > L.begin32[c])
+            e = class_blocks_n<float>(pl, c, L.begin32[c + 1] - L.begin32[c], &blocks32[c]);
+        if (e == hipSuccess && L.fb_cap[c] > 0) e = class_blocks_n<double>(pl, c, L.fb_cap[c], &blocks64[c]);
+    }
+    if (e != hipSuccess) return e;
+    if (pl->exec) {
+        (void)hipGraphExecDestroy(pl->exec);
+        pl->exec = nullptr;
+    }
+    if (pl->graph) {
+        (void)hipGraphDestroy(pl->graph);
+        pl->graph = nullptr;
+    }
+    unsigned *ctr = reinterpret_cast<unsigned *>(work);
+    float *a32 = reinterpret_cast<float *>(work + L.offa);
+    float *b32 = reinterpret_cast<float *>(work + L.offb);
+    int *info32 = reinterpret_cast<int *>(work + L.offinfo);
+    BatchProblem *fb = reinterpret_cast<BatchProblem *>(work + L.offfb);
+    const BatchProblem *t32 = reinterpret_cast<const BatchProblem *>(work + L.off32);
+    const cbk::MixedProblem *ref = reinterpret_cast<const cbk::MixedProblem *>(work + L.offref);
+    e = hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t le = cbk::launch_init_counters(ctr, reinterpret_cast<const unsigned *>(work + L.offinit),
+                                              kMixedCounters, pl->cap);
+    if (le == hipSuccess) {
+        const cbk::MixedPrepArgs pa{ap, bp, a32, b32, xp, ref, nullptr, (int)L.reftable.size(), iter};
+        le = cbk::launch_mixed_prep(pa, pl->cap);
+    }
+    if (le == hipSuccess) {
+        cbk::MixedPrepArgs pc{ap, bp, a32, b32, xp, nullptr,
+                              reinterpret_cast<const BatchProblem *>(work + L.offinel),
+                              (int)L.inel.size(), iter};
+        le = cbk::launch_mixed_prep(pc, pl->cap);
+    }
+    /* FP32 factor and solve of the refined problems */
+    if (le == hipSuccess) le = hipEventRecord(pl->fork, pl->cap);
+    for (int c = 0; c < kBatchKernels && le == hipSuccess; ++c) {
+        const int cnt = L.begin32[c + 1] - L.begin32[c];
+        if (cnt == 0) continue;
+        le = hipStreamWaitEvent(pl->side[c], pl->fork, 0);
+        const BatchArgs<float> a = list_args<float>(a32, b32, info32, t32 + L.begin32[c], cnt, ctr + c,
+                                                    nullptr, false, nullptr);
+        if (le == hipSuccess) le = launch_args<float>(c, a, blocks32[c], pl->side[c]);
+        if (le == hipSuccess) le = hipEventRecord(pl->join[c], pl->side[c]);
+        if (le == hipSuccess) le = hipStreamWaitEvent(pl->cap, pl->join[c], 0);
+    }
+    /* FP64 refinement, bucket by bucket */
+    for (int b = 0; b < cbk::kRefineBuckets && le == hipSuccess; ++b) {
+        cbk::MixedRefineArgs ra;
+        ra.a = ap;
+        ra.b = bp;
+        ra.x = xp;
+        ra.a32 = a32;
+        ra.b32 = b32;
+        ra.iter = iter;
+        ra.info = info;
+        ra.desc = ref + L.ref_begin[b];
+        ra.count = L.ref_begin[b + 1] - L.ref_begin[b];
+        ra.fb = fb;
+        ra.fb_fill = ctr + 2 * kBatchKernels;
+        for (int c = 0; c < kBatchKernels; ++c)
+            ra.fb_begin[c] = L.fb_begin[c];
+        ra.itermax = 30;
+        le = cbk::launch_mixed_refine(b, ra, pl->cap);
+    }
+    /* FP64 gels, in place on A and X, over the fallback lists */
+    if (le == hipSuccess) le = hipEventRecord(pl->fork2, pl->cap);
+    for (int c = 0; c < kBatchKernels && le == hipSuccess; ++c) {
+        if (L.fb_cap[c] == 0) continue;
+        le = hipStreamWaitEvent(pl->side[c], pl->fork2, 0);
+        const BatchArgs<double> a = list_args<double>(
+            ap, xp, info, fb + L.fb_begin[c], L.fb_cap[c], ctr + kBatchKernels + c,
+            reinterpret_cast<double *>(work + L.offtau), pl->tran, ctr + 2 * kBatchKernels + c);
+        if (le == hipSuccess) le = launch_args<double>(c, a, blocks64[c], pl->side[c]);
+        if (le == hipSuccess) le = hipEventRecord(pl->join2[c], pl->side[c]);
+        if (le == hipSuccess) le = hipStreamWaitEvent(pl->cap, pl->join2[c], 0);
+    }
+    e = hipStreamEndCapture(pl->cap, &pl->graph);
+    if (le != hipSuccess) return le;
+    if (e != hipSuccess) return e;
+    e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return e;
+    void *key[6] = {ap, const_cast<double *>(bp), xp, iter, info, work};
+    for (int i = 0; i < 6; ++i)
+        pl->mkey[i] = key[i];
+    return hipSuccess;
+}
+
+int dsgels_batch_impl(cbk_batch_plan *pl, double *ap, const double *bp, double *xp, int *iter,
+                      int *info, void *work, size_t lwork)
+{
+    if (!pl || pl->prec != 'M') return -1;
+    if (pl->nprob == 0) return 0;
+    if (!ap) return -2;
+    if (!bp) return -3;
+    if (!xp) return -4;
+    if (!iter) return -5;
+    if (!info) return -6;
+    if (!work && pl->bytes) return -7;
+    if (lwork < pl->bytes) return -8;
+    char *w = static_cast<char *>(work);
+    const hipStream_t st = cbk::current_stream();
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return cbk::launch_status(e);
+    if (dev != pl->device) return cbk::launch_status(hipErrorInvalidDevice);
+    MixedLayout &L = *pl->mx;
+    if (pl->uploaded != work) {
+        e = upload(w, L.off32, L.table32, st);
+        if (e == hipSuccess) e = upload(w, L.offfb, L.fbtable, st);
+        if (e == hipSuccess) e = upload(w, L.offref, L.reftable, st);
+        if (e == hipSuccess) e = upload(w, L.offinel, L.inel, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(w + L.offinit, L.init, sizeof(L.init), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return cbk::launch_status(e);
+        pl->uploaded = work;
+    }
+    void *key[6] = {ap, const_cast<double *>(bp), xp, iter, info, work};
+    bool same = pl->exec != nullptr;
+    for (int i = 0; i < 6; ++i)
+        same = same && pl->mkey[i] == key[i];
+    if (!same) {
+        e = record_graph_mixed(pl, ap, bp, xp, iter, info, w);
         if (e != hipSuccess) return cbk::launch_status(e);
     }
     return cbk::launch_status(hipGraphLaunch(pl->exec, st));
@@ -342,7 +636,7 @@ extern "C" {
 int cbk_gels_batch_plan(char prec, char trans, int nprob, const cbk_gels_problem *desc,
                         cbk_batch_plan_t *plan, int *bad)
 {
-    if (!opt(prec, 'D') && !opt(prec, 'S')) return -1;
+    if (!opt(prec, 'D') && !opt(prec, 'S') && !opt(prec, 'M')) return -1;
     if (!opt(trans, 'N') && !opt(trans, 'T') && !opt(trans, 'C')) return -2;
     if (nprob < 0) return -3;
     if (!desc && nprob > 0) return -4;
@@ -358,11 +652,21 @@ int cbk_gels_batch_plan(char prec, char trans, int nprob, const cbk_gels_problem
     }
     cbk_batch_plan *pl = new (std::nothrow) cbk_batch_plan;
     if (!pl) return cbk::launch_status(hipErrorOutOfMemory);
-    pl->prec = opt(prec, 'D') ? 'D' : 'S';
+    pl->prec = opt(prec, 'D') ? 'D' : opt(prec, 'S') ? 'S' : 'M';
     pl->tran = !opt(trans, 'N');
     pl->nprob = nprob;
     if (hipGetDevice(&pl->device) != hipSuccess) pl->device = 0;
     pl->tune = tuning_for(pl->device);
+    if (pl->prec == 'M') {
+        pl->mx = new (std::nothrow) MixedLayout;
+        if (!pl->mx) {
+            delete pl;
+            return cbk::launch_status(hipErrorOutOfMemory);
+        }
+        build_mixed(pl, desc);
+        *plan = pl;
+        return 0;
+    }
     std::vector<int> cls(nprob);
     std::vector<int> order(nprob);
     for (int k = 0; k < nprob; ++k) {
@@ -400,7 +704,8 @@ int cbk_batch_plan_classes(cbk_batch_plan_t plan, int *count)
     if (!plan || !count) return -1;
     for (int c = 0; c < kBatchClasses; ++c) count[c] = 0;
     for (int k = 0; k < kBatchKernels; ++k)
-        count[class_of_kernel(k)] += plan->begin[k + 1] - plan->begin[k];
+        count[class_of_kernel(k)] +=
+            plan->mx ? plan->mx->fb_cap[k] : plan->begin[k + 1] - plan->begin[k];
     return kBatchClasses;
 }
 
@@ -413,6 +718,12 @@ int dgels_batch(cbk_batch_plan_t plan, double *ap, double *bp, int *info, void *
 int sgels_batch(cbk_batch_plan_t plan, float *ap, float *bp, int *info, void *work, size_t lwork)
 {
     return gels_batch<float>(plan, 'S', ap, bp, info, work, lwork);
+}
+
+int dsgels_batch(cbk_batch_plan_t plan, double *ap, const double *bp, double *xp, int *iter,
+                 int *info, void *work, size_t lwork)
+{
+    return dsgels_batch_impl(plan, ap, bp, xp, iter, info, work, lwork);
 }
 
 int cbk_batch_plan_destroy(cbk_batch_plan_t plan)

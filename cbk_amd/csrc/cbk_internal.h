@@ -204,7 +204,17 @@ template <class T> struct BatchArgs {
     T *x;
     int x_rs, x_cs;
     long long x_gs;
+    /* optional: the number of problems at desc is *dcount (filled on the
+     * device before the launch; count is then the capacity the grid is
+     * sized for) */
+    const unsigned *dcount;
 };
+
+/* the problems a batch launch takes: *dcount when set, else count */
+template <class T> __device__ __forceinline__ unsigned batch_count(const BatchArgs<T> &p)
+{
+    return p.dcount ? *p.dcount : (unsigned)p.count;
+}
 
 /* classes: 0-2 team kernel with 16/32/64 row registers, 3-4 wavefront kernel
  * with 32/64, 5 the two-array wavefront kernel (n + nrhs > 64), 6 the general
@@ -228,6 +238,53 @@ hipError_t launch_qr_wave(int p, int q, int nrhs, bool transposed, Operand<T> A,
  * mask[k] < 0 only, B read only, X (q x nrhs) written; A receives the QR */
 hipError_t launch_qr_wave_masked(int p, int q, int nrhs, Operand<double> A, Operand<double> B,
                                  Operand<double> X, int V, int nm, const int *mask, int *info);
+
+/* ---- mixed-precision uneven batch (mixed_batch.hip, batch_plan.cpp) ---- */
+
+/* a problem of the FP32 path: the BatchProblem fields plus the FP64 kernel
+ * its fallback runs on */
+struct MixedProblem {
+    int m, n, nrhs, lda, ldb;
+    int idx;
+    long long a_off, b_off;
+    int k64;
+    int pad;
+};
+
+/* conv set: A32 / B32 := fl32(A / B) for the count problems at conv;
+ * otherwise X := B (max(m, n) x nrhs) and iter := -1 for the count problems
+ * at copy */
+struct MixedPrepArgs {
+    const double *a, *b;
+    float *a32, *b32;
+    double *x;
+    const MixedProblem *conv;
+    const BatchProblem *copy;
+    int count;
+    int *iter;
+};
+
+struct MixedRefineArgs {
+    const double *a, *b;
+    double *x;
+    const float *a32, *b32;
+    int *iter, *info;
+    const MixedProblem *desc;
+    int count;
+    BatchProblem *fb;           /* the FP64 fallback lists */
+    unsigned *fb_fill;          /* entries per list */
+    int fb_begin[kBatchKernels];
+    int itermax;
+};
+
+/* refinement kernels by shape: rows <= 32 / 64, columns <= 16 / 32 / 64,
+ * right-hand sides <= 4 / 8 */
+constexpr int kRefineBuckets = 10;
+int mixed_refine_bucket(int m, int n, int nrhs);
+hipError_t launch_mixed_refine(int bucket, const MixedRefineArgs &a, hipStream_t s);
+hipError_t launch_mixed_prep(const MixedPrepArgs &a, hipStream_t s);
+/* dst[0:n] := init[0:n] */
+hipError_t launch_init_counters(unsigned *dst, const unsigned *init, int n, hipStream_t s);
 
 /* ---- mixed-precision least squares (mixed_gels_impl.h) ---- */
 bool mixed_gels_supports(int p, int q, int nrhs);

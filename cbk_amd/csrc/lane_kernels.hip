@@ -449,7 +449,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void gels_batch_lane_kernel(BatchArgs<T> p)
 {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= p.count) return;
+    if (k >= (long)batch_count(p)) return;
     const BatchProblem d = p.desc[k];
     const LaneView<T> Av{p.a + d.a_off, 1, d.lda, 1};
     const LaneView<T> Bv{p.b + d.b_off, 1, d.ldb, 1};

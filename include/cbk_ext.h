@@ -194,10 +194,11 @@ typedef struct cbk_gels_problem {
 
 typedef struct cbk_batch_plan *cbk_batch_plan_t;
 
-/* prec 'D' (dgels_batch) or 'S' (sgels_batch); trans 'N', 'T' or 'C' as
- * gels_compact. Codes: -1 prec, -2 trans, -3 nprob < 0, -4 desc NULL,
- * -5 a descriptor is invalid (its index goes to *bad if bad is not NULL),
- * -6 plan NULL. On success *plan is a new plan (host memory only). */
+/* prec 'D' (dgels_batch), 'S' (sgels_batch) or 'M' (dsgels_batch); trans
+ * 'N', 'T' or 'C' as gels_compact. Codes: -1 prec, -2 trans, -3 nprob < 0,
+ * -4 desc NULL, -5 a descriptor is invalid (its index goes to *bad if bad is
+ * not NULL), -6 plan NULL. On success *plan is a new plan (host memory
+ * only). */
 int cbk_gels_batch_plan(char prec, char trans, int nprob, const cbk_gels_problem *desc,
                         cbk_batch_plan_t *plan, int *bad);
 
@@ -220,6 +221,32 @@ int dgels_batch(cbk_batch_plan_t plan, double *ap, double *bp, int *info, void *
                 size_t lwork);
 int sgels_batch(cbk_batch_plan_t plan, float *ap, float *bp, int *info, void *work,
                 size_t lwork);
+
+/* dsgels_batch: the batch in mixed precision (a plan made with prec 'M').
+ * A problem with trans 'N', 1 <= n <= m <= 64 and 1 <= nrhs <= 8 is solved by
+ * the FP32 batch gels (on FP32 copies of A and B in the workspace) and FP64
+ * refinement of that solution on the seminormal equations, the per-column
+ * scheme of dsgels_compact: r = B - A X and A^T r in FP64, R^T R D = A^T r
+ * with the FP32 R, X += D, until ||D||_inf <= ||X||_inf eps sqrt(n) (eps =
+ * 2^-53) or the correction stops halving once below ||X||_inf 2^-40. A
+ * problem that does not converge (a correction that does not contract, a
+ * NaN, a zero or non-finite diagonal of the FP32 R, or 30 steps), and every
+ * problem outside that range, is solved by the FP64 batch gels instead,
+ * in place on A and X. Everything runs as one recorded graph.
+ * A: read (left as it was for a refined problem; the FP64 QR for the rest,
+ * as dgels_batch). B: read only. X (xp): laid out as B (the same b_off and
+ * ldb, so a buffer the size of B); its first n rows of each problem receive
+ * the solution (the rows below are unspecified). Per problem (device ints):
+ *     iter[k] >= 0 refinement steps (the most any column took);
+ *             -1 outside the FP32 path's range, -31 refinement failed:
+ *             the FP64 path ran;
+ *     info[k] 0, or dgels_batch's code for a problem the FP64 path solved.
+ * Codes: -1 plan NULL or not a mixed plan, -2 ap, -3 bp, -4 xp, -5 iter,
+ * -6 info NULL, -7 work NULL with a nonzero worksize, -8 lwork too small;
+ * CBK_ERR_LAUNCH on a HIP error. The workspace holds the FP32 copies of A
+ * and B (as far as the largest offset of a refined problem reaches). */
+int dsgels_batch(cbk_batch_plan_t plan, double *ap, const double *bp, double *xp, int *iter,
+                 int *info, void *work, size_t lwork);
 
 int cbk_batch_plan_destroy(cbk_batch_plan_t plan);
 

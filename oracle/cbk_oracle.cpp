@@ -878,6 +878,64 @@ int gels_batch(char trans, int nprob, const oracle_gels_problem *desc, T *ap, T 
     }
     return 0;
 }
+/* the NaN-propagating max of the convergence tests */
+inline double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
+/* One column of the mixed least-squares refinement (dsgels_compact,
+ * dsgels_batch): x holds x_0 on entry (n values), the refined solution on
+ * exit; F's upper triangle is the single precision R. Per step
+ *     r = b - A x,  g = A^T r              (double, rows ascending)
+ *     R^T y = fl32(g) right-looking, R h = y by row dots   (float)
+ *     x += h
+ * until ||h||_inf <= ||x||_inf 2^-53 sqrt(n), or the correction stopped
+ * halving once below ||x||_inf 2^-40 (true); a correction that does not
+ * contract (a NaN included) or itermax steps give false. steps: the largest
+ * step index reached. */
+bool refine_column(long m, long n, const LV<double> &A, const LV<double> &B, long q,
+                   const LV<float> &F, std::vector<double> &x, int itermax, int &steps)
+{
+    const double cte = std::ldexp(1.0, -53) * std::sqrt((double)n), floor_ = std::ldexp(1.0, -40);
+    std::vector<double> gr(n);
+    std::vector<float> h(n);
+    double prev = INFINITY;
+    for (int it = 0; it <= itermax; ++it) {
+        for (long c = 0; c < n; ++c)
+            gr[c] = 0.0;
+        for (long i = 0; i < m; ++i) {
+            double r = B(i, q);
+            for (long c = 0; c < n; ++c)
+                r = std::fma(-x[c], A(i, c), r);
+            for (long c = 0; c < n; ++c)
+                gr[c] = std::fma(A(i, c), r, gr[c]);
+        }
+        for (long c = 0; c < n; ++c)
+            h[c] = (float)gr[c];
+        for (long k = 0; k < n; ++k) {
+            h[k] = h[k] / F(k, k);
+            for (long i = k + 1; i < n; ++i)
+                h[i] = std::fma(-F(k, i), h[k], h[i]);
+        }
+        for (long i = n - 1; i >= 0; --i) {
+            float s = h[i];
+            for (long k = i + 1; k < n; ++k)
+                s = std::fma(-F(i, k), h[k], s);
+            h[i] = s / F(i, i);
+        }
+        double dn = 0.0, xn = 0.0;
+        for (long i = 0; i < n; ++i) {
+            dn = nanmax(dn, std::fabs((double)h[i]));
+            x[i] = x[i] + (double)h[i];
+        }
+        for (long i = 0; i < n; ++i)
+            xn = nanmax(xn, std::fabs(x[i]));
+        steps = it > steps ? it : steps;
+        if (dn <= xn * cte || (it > 0 && dn > 0.5 * prev && dn <= xn * floor_)) return true;
+        if ((!(dn < prev) && it > 0) || it == itermax) return false;
+        prev = dn;
+    }
+    return false;
+}
+
 } /* namespace */
 
 extern "C" {
@@ -919,7 +977,6 @@ int oracle_dsgels_compact(char layout, char trans, int m, int n, int nrhs, doubl
                        ((m == 16 && n == 8) || (m == 32 && n == 16) || (m == 64 && n == 32));
     const size_t sa = group_stride(row, ldap, m, n, V), sb = group_stride(row, ldbp, m, nrhs, V),
                  sx = group_stride(row, ldxp, n, nrhs, V);
-    const double cte = std::ldexp(1.0, -53) * std::sqrt((double)n), floor_ = std::ldexp(1.0, -40);
     const int itermax = 30;
     for_each_lane(nm, V, [&](long g, long v) {
         const long idx = g * V + v;
@@ -939,53 +996,9 @@ int oracle_dsgels_compact(char layout, char trans, int m, int n, int nrhs, doubl
             for (long i = 0; i < n && ok; ++i)
                 ok = Fv(i, i) != 0.f && std::isfinite(Fv(i, i));
             for (long q = 0; q < nrhs && ok; ++q) {
-                std::vector<double> x(n, 0.0), gr(n);
-                std::vector<float> h(n);
-                double prev = INFINITY;
-                bool done = false;
-                for (int it = 0; it <= itermax; ++it) {
-                    for (long c = 0; c < n; ++c)
-                        gr[c] = 0.0;
-                    for (long i = 0; i < m; ++i) {
-                        double r = B(i, q);
-                        for (long c = 0; c < n; ++c)
-                            r = std::fma(-x[c], A(i, c), r);
-                        for (long c = 0; c < n; ++c)
-                            gr[c] = std::fma(A(i, c), r, gr[c]);
-                    }
-                    for (long c = 0; c < n; ++c)
-                        h[c] = (float)gr[c];
-                    for (long k = 0; k < n; ++k) {
-                        h[k] = h[k] / Fv(k, k);
-                        for (long i = k + 1; i < n; ++i)
-                            h[i] = std::fma(-Fv(k, i), h[k], h[i]);
-                    }
-                    for (long i = n - 1; i >= 0; --i) {
-                        float s = h[i];
-                        for (long k = i + 1; k < n; ++k)
-                            s = std::fma(-Fv(i, k), h[k], s);
-                        h[i] = s / Fv(i, i);
-                    }
-                    double dn = 0.0, xn = 0.0;
-                    for (long i = 0; i < n; ++i) {
-                        dn = std::fmax(dn, std::fabs((double)h[i]));
-                        x[i] = x[i] + (double)h[i];
-                    }
-                    for (long i = 0; i < n; ++i)
-                        xn = std::fmax(xn, std::fabs(x[i]));
-                    if (std::isnan(dn)) break;
-                    steps = it > steps ? it : steps;
-                    if (dn <= xn * cte || (it > 0 && dn > 0.5 * prev && dn <= xn * floor_)) {
-                        done = true;
-                        break;
-                    }
-                    if (!(dn < prev) && it > 0) break;
-                    prev = dn;
-                }
-                if (!done) {
-                    ok = false;
-                    break;
-                }
+                std::vector<double> x(n, 0.0);
+                ok = refine_column(m, n, A, B, q, Fv, x, itermax, steps);
+                if (!ok) break;
                 for (long i = 0; i < n; ++i)
                     X(i, q) = x[i];
             }
@@ -1028,6 +1041,82 @@ int oracle_sgels_batch(char trans, int nprob, const oracle_gels_problem *desc, f
                        float *bp, int *info)
 {
     return gels_batch<float>(trans, nprob, desc, ap, bp, info);
+}
+
+/* The mixed-precision uneven batch (the library's dsgels_batch; no reference
+ * counterpart). A problem with trans 'N', 1 <= n <= m <= 64 and
+ * 1 <= nrhs <= 8: the single precision gels (this file's gels_lane in float,
+ * the reference's gels_compact_group, src/cbk_gels_compact.hpp:84-106) on
+ * float copies of A and B gives R and x_0; each column is then refined as
+ * dsgels_compact's (refine_column) from x_0. A problem that does not converge
+ * (or has a zero or non-finite R(i,i)) gets iter = -31, every other problem
+ * iter = -1, and both are solved by the double gels in place on A and on X
+ * := B. X is laid out as B. */
+int oracle_dsgels_batch(char trans, int nprob, const oracle_gels_problem *desc, double *ap,
+                        const double *bp, double *xp, int *iter, int *info)
+{
+    if (!opt(trans, 'N') && !opt(trans, 'T') && !opt(trans, 'C')) return -1;
+    if (nprob < 0) return -2;
+    const bool tran = !opt(trans, 'N');
+    const int itermax = 30;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int k = 0; k < nprob; ++k) {
+        const oracle_gels_problem &d = desc[k];
+        const long m = d.m, n = d.n, nrhs = d.nrhs;
+        const LV<double> A = make_view(ap + d.a_off, false, d.lda, 1);
+        const LV<double> B = make_view(const_cast<double *>(bp) + d.b_off, false, d.ldb, 1);
+        const LV<double> X = make_view(xp + d.b_off, false, d.ldb, 1);
+        const bool mixed = !tran && n >= 1 && n <= m && m <= 64 && nrhs >= 1 && nrhs <= 8;
+        bool ok = mixed;
+        int steps = 0;
+        info[k] = 0;
+        if (mixed) {
+            std::vector<float> F((size_t)m * n), S((size_t)m * nrhs), tau(n);
+            const LV<float> Fv{F.data(), 1, m, 1}, Sv{S.data(), 1, m, 1};
+            for (long j = 0; j < n; ++j)
+                for (long i = 0; i < m; ++i)
+                    F[j * m + i] = (float)A(i, j);
+            for (long q = 0; q < nrhs; ++q)
+                for (long i = 0; i < m; ++i)
+                    S[q * m + i] = (float)B(i, q);
+            gels_lane<float>(true, m, n, nrhs, Fv, tau.data(), Sv, 1, nullptr);
+            for (long i = 0; i < n && ok; ++i)
+                ok = Fv(i, i) != 0.f && std::isfinite(Fv(i, i));
+            std::vector<std::vector<double>> xs(nrhs, std::vector<double>(n));
+            for (long q = 0; q < nrhs && ok; ++q) {
+                for (long i = 0; i < n; ++i)
+                    xs[q][i] = (double)Sv(i, q);
+                ok = refine_column(m, n, A, B, q, Fv, xs[q], itermax, steps);
+            }
+            if (ok) {
+                for (long q = 0; q < nrhs; ++q)
+                    for (long i = 0; i < n; ++i)
+                        X(i, q) = xs[q][i];
+                iter[k] = steps;
+                continue;
+            }
+        }
+        iter[k] = mixed ? -itermax - 1 : -1;
+        const long mx = m > n ? m : n;
+        for (long q = 0; q < nrhs; ++q)
+            for (long i = 0; i < mx; ++i)
+                X(i, q) = B(i, q);
+        const bool tall = m >= n;
+        const long p = tall ? m : n, qq = tall ? n : m;
+        int bad = 0;
+        if (nrhs > 0 && mx > 0) {
+            if (qq == 0) {
+                zero_block(X, 0, p, nrhs);
+            }
+            else {
+                std::vector<double> tau(qq);
+                gels_lane<double>(tall != tran, p, qq, nrhs, tall ? A : A.tr(), tau.data(), X, 1,
+                                  &bad);
+            }
+        }
+        info[k] = bad;
+    }
+    return 0;
 }
 
 } /* extern "C" */

@@ -139,6 +139,15 @@ int oracle_dgels_batch(char trans, int nprob, const oracle_gels_problem *desc, d
 int oracle_sgels_batch(char trans, int nprob, const oracle_gels_problem *desc, float *ap,
                        float *bp, int *info);
 
+/* The library's dsgels_batch restated per problem (no reference
+ * counterpart): single precision gels, then double refinement of its
+ * solution on the seminormal equations (dsgels_compact's per-column scheme),
+ * the double gels in place on A and X := B for the problems that do not
+ * converge (iter = -31) or that have no single precision path (-1). X is
+ * laid out as B; iter and info per problem. */
+int oracle_dsgels_batch(char trans, int nprob, const oracle_gels_problem *desc, double *ap,
+                        const double *bp, double *xp, int *iter, int *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,3 +105,53 @@ def check_ls(p, desc, As, Bs, ap_o, bp_o, ap_g, bp_g, factor=20.0):
         worst = max(worst, ef, ex, er)
         assert ef <= 1 and ex <= 1 and er <= 1, (k, d, ef, ex, er)
     return worst
+
+
+def set_a(ap, d, A):
+    m, n, nrhs, lda, ldb, a0, b0 = d
+    for j in range(n):
+        ap[a0 + j * lda: a0 + j * lda + m] = A[:, j]
+
+
+def ill_conditioned(r, m, n, logk=12):
+    """m x n, singular values 10^(-logk * U[0,1]) with 1 and 10^-logk present"""
+    U = np.linalg.qr(r.standard_normal((m, n)))[0]
+    W = np.linalg.qr(r.standard_normal((n, n)))[0]
+    s = 10.0 ** (-r.uniform(0, 1, n) * logk)
+    s[0], s[-1] = 1.0, 10.0 ** -logk
+    return (U * s) @ W.T
+
+
+def oracle_mixed(trans, desc, ap, bp):
+    """oracle_dsgels_batch on copies: (A after, X, iter, info)"""
+    arr = (OracleProblem * max(1, len(desc)))()
+    for i, d in enumerate(desc):
+        arr[i] = OracleProblem(*d)
+    ap = ap.copy()
+    xp = np.full_like(bp, 7.0)
+    it = np.zeros(len(desc), np.int32)
+    info = np.zeros(len(desc), np.int32)
+    f = cbkt.oracle().lib.oracle_dsgels_batch
+    f.restype = ctypes.c_int
+    st = f(ctypes.c_char(trans.encode()), ctypes.c_int(len(desc)), arr,
+           ap.ctypes.data_as(ctypes.c_void_p), bp.ctypes.data_as(ctypes.c_void_p),
+           xp.ctypes.data_as(ctypes.c_void_p), it.ctypes.data_as(ctypes.c_void_p),
+           info.ctypes.data_as(ctypes.c_void_p))
+    assert st == 0
+    return ap, xp, it, info
+
+
+def mixed_mix(r, nuneven=200):
+    """a batch for the mixed path: shapes it refines (every refinement bucket),
+    problems it must hand to FP64 (cond 1e12), and shapes outside its range;
+    returns (shapes, indices of the ill-conditioned, indices outside)"""
+    shapes = [
+        (16, 8, 2), (12, 5, 3), (32, 16, 5), (30, 30, 1),          # rows <= 32, n <= 16 / 32
+        (64, 8, 8), (50, 13, 4), (64, 32, 2), (40, 33, 7),         # rows <= 64, n <= 16 / 32 / 64
+        (64, 64, 8), (64, 50, 3), (4, 2, 1), (2, 1, 1),
+    ] + uneven_shapes(r, nuneven)
+    ill = list(range(len(shapes), len(shapes) + 4))
+    shapes += [(24, 12, 2), (64, 40, 3), (32, 32, 6), (48, 20, 8)]
+    outside = list(range(len(shapes), len(shapes) + 6))
+    shapes += [(70, 10, 2), (30, 40, 2), (20, 5, 9), (5, 0, 2), (6, 3, 0), (80, 64, 4)]
+    return shapes, ill, outside
