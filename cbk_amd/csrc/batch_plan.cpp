@@ -10,13 +10,14 @@
  *   then, 256-aligned        tau scratch of the general kernel's problems
  *
  * A mixed plan (dsgels_batch) keeps, each 256-aligned after the counters
- * (FP32 class queues, FP64 class queues, FP64 list fills): the FP32 table of
- * the problems the FP32 path serves, the FP64 fallback lists (one segment
- * per kernel, sized for every problem of that kernel, the problems without
- * an FP32 path placed at their starts at plan time), the refinement table
- * grouped by shape bucket, the list of the problems without an FP32 path,
- * the counters' initial values, the FP32 info scratch, the FP64 tau scratch
- * and the FP32 copies of A and B (at the caller's offsets).
+ * (FP32 class queues, the FP64 queues of the problems without an FP32 path
+ * and of the fallback lists, the fallback lists' fills): the FP32 table of
+ * the problems the FP32 path serves, the FP64 lists (one segment per kernel:
+ * the problems without an FP32 path, fixed at plan time, then room for that
+ * kernel's refined problems that fall back), the refinement table grouped
+ * by shape bucket, the list of the problems without an FP32 path, the FP32
+ * info scratch, the FP64 tau scratch and the FP32 copies of A and B (at the
+ * caller's offsets).
  */
 
 #include "cbk.h"
@@ -101,27 +102,32 @@ int class_of_kernel(int k) { return k >= kBatchClasses ? k - 4 : k; }
 
 int problems_per_block(int k) { return k <= 2 ? 16 : k <= 5 ? 4 : k == kBatchGeneral ? 256 : 8; }
 
-/* a problem the mixed path refines: trans 'N', 1 <= n <= m <= 64,
+/* a problem the mixed path refines: trans 'N', 1 <= n <= m <= 64, n <= 32,
  * 1 <= nrhs <= 8 */
 bool mixed_eligible(const cbk_gels_problem &p, bool tran)
 {
-    return !tran && p.n >= 1 && p.n <= p.m && p.m <= 64 && p.nrhs >= 1 && p.nrhs <= 8;
+    return !tran && p.n >= 1 && p.n <= p.m && p.m <= 64 && p.n <= 32 && p.nrhs >= 1 &&
+           p.nrhs <= 8;
 }
 
-constexpr int kMixedCounters = 3 * kBatchKernels; /* FP32 queues, FP64 queues, list fills */
+/* FP32 queues, FP64 queues of the problems without an FP32 path, FP64
+ * queues of the fallback lists, the fallback lists' fills */
+constexpr int kMixedCounters = 4 * kBatchKernels;
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
+/* fbtable, per FP64 kernel c: the nstat[c] problems without an FP32 path
+ * (fixed at plan time) from fb_begin[c], then room for the ndyn[c] refined
+ * problems of kernel c that may fall back (filled on the device) */
 struct MixedLayout {
     std::vector<BatchProblem> table32, fbtable, inel;
     std::vector<cbk::MixedProblem> reftable;
     int begin32[kBatchKernels + 1] = {};
-    int fb_begin[kBatchKernels] = {}, fb_cap[kBatchKernels] = {};
-    unsigned init[kMixedCounters] = {};
+    int fb_begin[kBatchKernels] = {}, nstat[kBatchKernels] = {}, ndyn[kBatchKernels] = {};
     int ref_begin[cbk::kRefineBuckets + 1] = {};
     long long ext_a = 0, ext_b = 0, tau64 = 0;
-    size_t off32 = 0, offfb = 0, offref = 0, offinel = 0, offinit = 0, offinfo = 0, offtau = 0,
-           offa = 0, offb = 0;
+    size_t off32 = 0, offfb = 0, offref = 0, offinel = 0, offinfo = 0, offtau = 0, offa = 0,
+           offb = 0;
 };
 
 } /* namespace */
@@ -147,6 +153,8 @@ struct cbk_batch_plan {
     MixedLayout *mx = nullptr;
     void *mkey[6] = {};
     hipEvent_t fork2 = nullptr, join2[kBatchKernels] = {};
+    hipEvent_t fork3 = nullptr, mjoin[kBatchKernels] = {}, rjoin[cbk::kRefineBuckets] = {};
+    hipStream_t mside[kBatchKernels] = {}, rside[cbk::kRefineBuckets] = {};
 
     ~cbk_batch_plan()
     {
@@ -158,9 +166,17 @@ struct cbk_batch_plan {
         }
         if (cap) (void)hipStreamDestroy(cap);
         if (fork) (void)hipEventDestroy(fork);
-        for (int c = 0; c < kBatchKernels; ++c)
+        for (int c = 0; c < kBatchKernels; ++c) {
             if (join2[c]) (void)hipEventDestroy(join2[c]);
+            if (mjoin[c]) (void)hipEventDestroy(mjoin[c]);
+            if (mside[c]) (void)hipStreamDestroy(mside[c]);
+        }
+        for (int b = 0; b < cbk::kRefineBuckets; ++b) {
+            if (rjoin[b]) (void)hipEventDestroy(rjoin[b]);
+            if (rside[b]) (void)hipStreamDestroy(rside[b]);
+        }
         if (fork2) (void)hipEventDestroy(fork2);
+        if (fork3) (void)hipEventDestroy(fork3);
         delete mx;
     }
 };
@@ -182,6 +198,22 @@ hipError_t ensure_streams(cbk_batch_plan *pl)
 }
 
 /* the arguments of one class launch over count problems at desc */
+/* the extra branches of a mixed plan's graph */
+hipError_t ensure_mixed_streams(cbk_batch_plan *pl)
+{
+    if (pl->fork3) return hipSuccess;
+    hipError_t e = hipEventCreateWithFlags(&pl->fork3, hipEventDisableTiming);
+    for (int c = 0; c < kBatchKernels && e == hipSuccess; ++c) {
+        e = hipStreamCreateWithFlags(&pl->mside[c], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->mjoin[c], hipEventDisableTiming);
+    }
+    for (int b = 0; b < cbk::kRefineBuckets && e == hipSuccess; ++b) {
+        e = hipStreamCreateWithFlags(&pl->rside[b], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->rjoin[b], hipEventDisableTiming);
+    }
+    return e;
+}
+
 template <class T>
 BatchArgs<T> list_args(T *ap, T *bp, int *info, const BatchProblem *desc, int count,
                        unsigned *counter, T *tau, bool tran, const unsigned *dcount)
@@ -342,25 +374,31 @@ void build_mixed(cbk_batch_plan *pl, const cbk_gels_problem *desc)
     for (int k = 0; k < np; ++k) {
         k64[k] = kernel_of(classify(desc[k], pl->tran, pl->tune), desc[k]);
         el[k] = mixed_eligible(desc[k], pl->tran);
-        L.fb_cap[k64[k]]++;
+        (el[k] ? L.ndyn : L.nstat)[k64[k]]++;
     }
     for (int c = 1; c < kBatchKernels; ++c)
-        L.fb_begin[c] = L.fb_begin[c - 1] + L.fb_cap[c - 1];
+        L.fb_begin[c] = L.fb_begin[c - 1] + L.nstat[c - 1] + L.ndyn[c - 1];
     L.fbtable.assign(np, BatchProblem{0, 0, 0, 1, 1, 0, 0, 0, 0});
-    std::vector<int> order;
-    for (int k = 0; k < np; ++k) {
+    std::vector<int> order, placed(kBatchKernels, 0);
+    /* the problems without an FP32 path, each kernel's longest first */
+    std::vector<int> stat;
+    for (int k = 0; k < np; ++k)
+        if (!el[k]) stat.push_back(k);
+    std::stable_sort(stat.begin(), stat.end(),
+                     [&](int x, int y) { return work_of(desc[x]) > work_of(desc[y]); });
+    for (int k : stat) {
         const cbk_gels_problem &p = desc[k];
         BatchProblem b{p.m, p.n, p.nrhs, p.lda, p.ldb, k, p.a_off, p.b_off, 0};
-        if (!el[k]) {
-            /* no FP32 path: the FP64 gels runs on it directly */
-            if (k64[k] == kBatchGeneral) {
-                b.tau_off = L.tau64;
-                L.tau64 += std::min(p.m, p.n);
-            }
-            L.fbtable[L.fb_begin[k64[k]] + L.init[2 * kBatchKernels + k64[k]]++] = b;
-            L.inel.push_back(b);
-            continue;
+        if (k64[k] == kBatchGeneral) {
+            b.tau_off = L.tau64;
+            L.tau64 += std::min(p.m, p.n);
         }
+        L.fbtable[L.fb_begin[k64[k]] + placed[k64[k]]++] = b;
+        L.inel.push_back(b);
+    }
+    for (int k = 0; k < np; ++k) {
+        const cbk_gels_problem &p = desc[k];
+        if (!el[k]) continue;
         order.push_back(k);
         L.ext_a = std::max(L.ext_a, p.a_off + (long long)p.lda * (p.n - 1) + p.m);
         L.ext_b = std::max(L.ext_b, p.b_off + (long long)p.ldb * (p.nrhs - 1) + p.m);
@@ -402,8 +440,6 @@ void build_mixed(cbk_batch_plan *pl, const cbk_gels_problem *desc)
     off = align256(off + L.reftable.size() * sizeof(cbk::MixedProblem));
     L.offinel = off;
     off = align256(off + L.inel.size() * sizeof(BatchProblem));
-    L.offinit = off;
-    off = align256(off + sizeof(L.init));
     L.offinfo = off;
     off = align256(off + (size_t)np * sizeof(int));
     L.offtau = off;
@@ -422,20 +458,24 @@ hipError_t upload(char *w, size_t off, const std::vector<E> &v, hipStream_t s)
     return hipMemcpyAsync(w + off, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice, s);
 }
 
-/* counters from their initial values, the conversion and copies, the FP32
- * classes on their branches, the refinement buckets, then the FP64 classes
- * over the fallback lists on their branches */
+/* Counters cleared; the conversion and copies; then on their own branches,
+ * concurrently: every FP32 class followed (after all of them) by the
+ * refinement buckets, and the FP64 classes of the problems without an FP32
+ * path; after the refinement, the FP64 classes over the fallback lists. */
 hipError_t record_graph_mixed(cbk_batch_plan *pl, double *ap, const double *bp, double *xp,
                               int *iter, int *info, char *work)
 {
     MixedLayout &L = *pl->mx;
+    constexpr int K = kBatchKernels;
     hipError_t e = ensure_streams(pl);
+    if (e == hipSuccess) e = ensure_mixed_streams(pl);
     if (e != hipSuccess) return e;
-    unsigned blocks32[kBatchKernels] = {}, blocks64[kBatchKernels] = {};
-    for (int c = 0; c < kBatchKernels && e == hipSuccess; ++c) {
+    unsigned b32[K] = {}, bst[K] = {}, bdyn[K] = {};
+    for (int c = 0; c < K && e == hipSuccess; ++c) {
         if (L.begin32[c + 1] > L.begin32[c])
-            e = class_blocks_n<float>(pl, c, L.begin32[c + 1] - L.begin32[c], &blocks32[c]);
-        if (e == hipSuccess && L.fb_cap[c] > 0) e = class_blocks_n<double>(pl, c, L.fb_cap[c], &blocks64[c]);
+            e = class_blocks_n<float>(pl, c, L.begin32[c + 1] - L.begin32[c], &b32[c]);
+        if (e == hipSuccess && L.nstat[c] > 0) e = class_blocks_n<double>(pl, c, L.nstat[c], &bst[c]);
+        if (e == hipSuccess && L.ndyn[c] > 0) e = class_blocks_n<double>(pl, c, L.ndyn[c], &bdyn[c]);
     }
     if (e != hipSuccess) return e;
     if (pl->exec) {
@@ -447,70 +487,91 @@ hipError_t record_graph_mixed(cbk_batch_plan *pl, double *ap, const double *bp, 
         pl->graph = nullptr;
     }
     unsigned *ctr = reinterpret_cast<unsigned *>(work);
+    unsigned *q32 = ctr, *qst = ctr + K, *qdyn = ctr + 2 * K, *fill = ctr + 3 * K;
     float *a32 = reinterpret_cast<float *>(work + L.offa);
-    float *b32 = reinterpret_cast<float *>(work + L.offb);
+    float *b32p = reinterpret_cast<float *>(work + L.offb);
     int *info32 = reinterpret_cast<int *>(work + L.offinfo);
+    double *tau = reinterpret_cast<double *>(work + L.offtau);
     BatchProblem *fb = reinterpret_cast<BatchProblem *>(work + L.offfb);
     const BatchProblem *t32 = reinterpret_cast<const BatchProblem *>(work + L.off32);
     const cbk::MixedProblem *ref = reinterpret_cast<const cbk::MixedProblem *>(work + L.offref);
-    e = hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal);
+    hipStream_t cap = pl->cap;
+    e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    hipError_t le = cbk::launch_init_counters(ctr, reinterpret_cast<const unsigned *>(work + L.offinit),
-                                              kMixedCounters, pl->cap);
+    hipError_t le = cbk::launch_clear_u32(ctr, kMixedCounters, cap);
     if (le == hipSuccess) {
-        const cbk::MixedPrepArgs pa{ap, bp, a32, b32, xp, ref, nullptr, (int)L.reftable.size(), iter};
-        le = cbk::launch_mixed_prep(pa, pl->cap);
+        const cbk::MixedPrepArgs pa{ap, bp, a32, b32p, xp, ref, nullptr, (int)L.reftable.size(), iter};
+        le = cbk::launch_mixed_prep(pa, cap);
     }
     if (le == hipSuccess) {
-        cbk::MixedPrepArgs pc{ap, bp, a32, b32, xp, nullptr,
-                              reinterpret_cast<const BatchProblem *>(work + L.offinel),
-                              (int)L.inel.size(), iter};
-        le = cbk::launch_mixed_prep(pc, pl->cap);
+        const cbk::MixedPrepArgs pc{ap, bp, a32, b32p, xp, nullptr,
+                                    reinterpret_cast<const BatchProblem *>(work + L.offinel),
+                                    (int)L.inel.size(), iter};
+        le = cbk::launch_mixed_prep(pc, cap);
     }
-    /* FP32 factor and solve of the refined problems */
-    if (le == hipSuccess) le = hipEventRecord(pl->fork, pl->cap);
-    for (int c = 0; c < kBatchKernels && le == hipSuccess; ++c) {
+    if (le == hipSuccess) le = hipEventRecord(pl->fork, cap);
+    /* the FP64 problems without an FP32 path: independent of everything after */
+    for (int c = 0; c < K && le == hipSuccess; ++c) {
+        if (L.nstat[c] == 0) continue;
+        hipStream_t s = pl->mside[c];
+        le = hipStreamWaitEvent(s, pl->fork, 0);
+        const BatchArgs<double> a = list_args<double>(ap, xp, info, fb + L.fb_begin[c], L.nstat[c],
+                                                      qst + c, tau, pl->tran, nullptr);
+        if (le == hipSuccess) le = launch_args<double>(c, a, bst[c], s);
+        if (le == hipSuccess) le = hipEventRecord(pl->mjoin[c], s);
+    }
+    /* the FP32 classes */
+    for (int c = 0; c < K && le == hipSuccess; ++c) {
         const int cnt = L.begin32[c + 1] - L.begin32[c];
         if (cnt == 0) continue;
         le = hipStreamWaitEvent(pl->side[c], pl->fork, 0);
-        const BatchArgs<float> a = list_args<float>(a32, b32, info32, t32 + L.begin32[c], cnt, ctr + c,
-                                                    nullptr, false, nullptr);
-        if (le == hipSuccess) le = launch_args<float>(c, a, blocks32[c], pl->side[c]);
+        const BatchArgs<float> a = list_args<float>(a32, b32p, info32, t32 + L.begin32[c], cnt,
+                                                    q32 + c, nullptr, false, nullptr);
+        if (le == hipSuccess) le = launch_args<float>(c, a, b32[c], pl->side[c]);
         if (le == hipSuccess) le = hipEventRecord(pl->join[c], pl->side[c]);
-        if (le == hipSuccess) le = hipStreamWaitEvent(pl->cap, pl->join[c], 0);
+        if (le == hipSuccess) le = hipStreamWaitEvent(cap, pl->join[c], 0);
     }
-    /* FP64 refinement, bucket by bucket */
+    /* the refinement buckets, on branches of their own */
+    if (le == hipSuccess) le = hipEventRecord(pl->fork2, cap);
     for (int b = 0; b < cbk::kRefineBuckets && le == hipSuccess; ++b) {
+        const int cnt = L.ref_begin[b + 1] - L.ref_begin[b];
+        if (cnt == 0) continue;
+        hipStream_t s = pl->rside[b];
+        le = hipStreamWaitEvent(s, pl->fork2, 0);
         cbk::MixedRefineArgs ra;
         ra.a = ap;
         ra.b = bp;
         ra.x = xp;
         ra.a32 = a32;
-        ra.b32 = b32;
+        ra.b32 = b32p;
         ra.iter = iter;
         ra.info = info;
         ra.desc = ref + L.ref_begin[b];
-        ra.count = L.ref_begin[b + 1] - L.ref_begin[b];
+        ra.count = cnt;
         ra.fb = fb;
-        ra.fb_fill = ctr + 2 * kBatchKernels;
-        for (int c = 0; c < kBatchKernels; ++c)
-            ra.fb_begin[c] = L.fb_begin[c];
+        ra.fb_fill = fill;
+        for (int c = 0; c < K; ++c)
+            ra.fb_begin[c] = L.fb_begin[c] + L.nstat[c];
         ra.itermax = 30;
-        le = cbk::launch_mixed_refine(b, ra, pl->cap);
+        if (le == hipSuccess) le = cbk::launch_mixed_refine(b, ra, s);
+        if (le == hipSuccess) le = hipEventRecord(pl->rjoin[b], s);
+        if (le == hipSuccess) le = hipStreamWaitEvent(cap, pl->rjoin[b], 0);
     }
     /* FP64 gels, in place on A and X, over the fallback lists */
-    if (le == hipSuccess) le = hipEventRecord(pl->fork2, pl->cap);
-    for (int c = 0; c < kBatchKernels && le == hipSuccess; ++c) {
-        if (L.fb_cap[c] == 0) continue;
-        le = hipStreamWaitEvent(pl->side[c], pl->fork2, 0);
-        const BatchArgs<double> a = list_args<double>(
-            ap, xp, info, fb + L.fb_begin[c], L.fb_cap[c], ctr + kBatchKernels + c,
-            reinterpret_cast<double *>(work + L.offtau), pl->tran, ctr + 2 * kBatchKernels + c);
-        if (le == hipSuccess) le = launch_args<double>(c, a, blocks64[c], pl->side[c]);
+    if (le == hipSuccess) le = hipEventRecord(pl->fork3, cap);
+    for (int c = 0; c < K && le == hipSuccess; ++c) {
+        if (L.ndyn[c] == 0) continue;
+        le = hipStreamWaitEvent(pl->side[c], pl->fork3, 0);
+        const BatchArgs<double> a =
+            list_args<double>(ap, xp, info, fb + L.fb_begin[c] + L.nstat[c], L.ndyn[c], qdyn + c,
+                              tau, pl->tran, fill + c);
+        if (le == hipSuccess) le = launch_args<double>(c, a, bdyn[c], pl->side[c]);
         if (le == hipSuccess) le = hipEventRecord(pl->join2[c], pl->side[c]);
-        if (le == hipSuccess) le = hipStreamWaitEvent(pl->cap, pl->join2[c], 0);
+        if (le == hipSuccess) le = hipStreamWaitEvent(cap, pl->join2[c], 0);
     }
-    e = hipStreamEndCapture(pl->cap, &pl->graph);
+    for (int c = 0; c < K && le == hipSuccess; ++c)
+        if (L.nstat[c] > 0) le = hipStreamWaitEvent(cap, pl->mjoin[c], 0);
+    e = hipStreamEndCapture(cap, &pl->graph);
     if (le != hipSuccess) return le;
     if (e != hipSuccess) return e;
     e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0);
@@ -545,8 +606,6 @@ int dsgels_batch_impl(cbk_batch_plan *pl, double *ap, const double *bp, double *
         if (e == hipSuccess) e = upload(w, L.offfb, L.fbtable, st);
         if (e == hipSuccess) e = upload(w, L.offref, L.reftable, st);
         if (e == hipSuccess) e = upload(w, L.offinel, L.inel, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(w + L.offinit, L.init, sizeof(L.init), hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return cbk::launch_status(e);
         pl->uploaded = work;
     }
@@ -705,7 +764,7 @@ int cbk_batch_plan_classes(cbk_batch_plan_t plan, int *count)
     for (int c = 0; c < kBatchClasses; ++c) count[c] = 0;
     for (int k = 0; k < kBatchKernels; ++k)
         count[class_of_kernel(k)] +=
-            plan->mx ? plan->mx->fb_cap[k] : plan->begin[k + 1] - plan->begin[k];
+            plan->mx ? plan->mx->nstat[k] + plan->mx->ndyn[k] : plan->begin[k + 1] - plan->begin[k];
     return kBatchClasses;
 }
 

@@ -273,18 +273,16 @@ struct MixedRefineArgs {
     int count;
     BatchProblem *fb;           /* the FP64 fallback lists */
     unsigned *fb_fill;          /* entries per list */
-    int fb_begin[kBatchKernels];
+    int fb_begin[kBatchKernels]; /* where each kernel's list starts in fb */
     int itermax;
 };
 
-/* refinement kernels by shape: rows <= 32 / 64, columns <= 16 / 32 / 64,
+/* refinement kernels by shape: rows <= 32 / 64, columns <= 16 / 32,
  * right-hand sides <= 4 / 8 */
-constexpr int kRefineBuckets = 10;
+constexpr int kRefineBuckets = 8;
 int mixed_refine_bucket(int m, int n, int nrhs);
 hipError_t launch_mixed_refine(int bucket, const MixedRefineArgs &a, hipStream_t s);
 hipError_t launch_mixed_prep(const MixedPrepArgs &a, hipStream_t s);
-/* dst[0:n] := init[0:n] */
-hipError_t launch_init_counters(unsigned *dst, const unsigned *init, int n, hipStream_t s);
 
 /* ---- mixed-precision least squares (mixed_gels_impl.h) ---- */
 bool mixed_gels_supports(int p, int q, int nrhs);

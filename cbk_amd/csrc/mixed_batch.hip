@@ -13,24 +13,25 @@
  *   (the FP64 batch gels over the fallback lists, in place on A and X)
  *
  * Refinement (the per-column scheme of dsgels_compact, started from the FP32
- * solution x_0): per step
+ * solution x_0), with R^-1 formed once per problem in FP32 (column j by back
+ * substitution, in lane j) so that no step waits on a pivot chain: per step
  *     r = b - A x          FP64, per row, fma over columns ascending
  *     g = A^T r            FP64, per column, fma over rows ascending
- *     R^T R h = fl32(g)    FP32: forward sweep right-looking (per element in
- *                          the order of the oracle's), backward sweep
- *                          right-looking from the last row
+ *     y = R^-T fl32(g)     FP32, lane j sums k <= j ascending
+ *     h = R^-1 y           FP32, lane i sums k >= i ascending
  *     x += h
  * until ||h||_inf <= ||x||_inf eps sqrt(n) (eps = 2^-53), or the correction
  * stopped halving once below ||x||_inf 2^-40; a correction that does not
  * contract, a NaN, a zero or non-finite R(k,k) or 30 steps send the problem
  * to FP64.
  *
- * Layout: A (m <= MB rows, n <= NB columns) is staged once in LDS, row-major
- * with a padded stride, so the residual (lane = row) and A^T r (lane =
- * column) both read it conflict-free with no cross-lane reductions; R sits in
- * LDS beside it; the solves run with lane = (row group, right-hand side),
- * the pivot broadcast by ds_bpermute. HBM traffic per problem: A, B and the
- * FP32 R and x_0 read once, X written once.
+ * Layout: lane i holds row i of A in registers for the residual; A is also
+ * staged once in LDS (row-major, padded stride) for A^T r, lane = column;
+ * lane j holds column j and row j of R^-1 for the two triangular products;
+ * x, r, g and y pass between the phases through LDS as broadcasts, so there
+ * are no cross-lane reductions. Column q's convergence is decided by lane q
+ * over the |h| and |x| the lanes leave in LDS. HBM traffic per problem: A, B
+ * and the FP32 R and x_0 read once, X written once.
  */
 
 #include "cbk_internal.h"
@@ -85,26 +86,19 @@ __global__ __launch_bounds__(256) void mixed_prep_kernel(MixedPrepArgs p)
     }
 }
 
-/* counters := their initial values (no memset node in the graph) */
-__global__ void init_counters_kernel(unsigned *dst, const unsigned *init, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = init[i];
-}
-
 /* ---- refinement -------------------------------------------------------------- */
 
 template <int MB, int NB, int Q>
 __global__ __launch_bounds__(64) void mixed_refine_kernel(MixedRefineArgs p)
 {
-    constexpr int SA = NB + 1;      /* LDS row stride of A and R */
-    constexpr int G = 64 / Q;       /* row groups of the solve layout */
-    constexpr int T = (NB + G - 1) / G;
-    __shared__ double sA[MB * SA];
+    constexpr int SA = NB + 1; /* LDS row stride of A, R and R^-1 */
+    __shared__ double sA[MB * SA];  /* A(i, c) for the A^T r pass */
     __shared__ double sR[MB * Q];   /* r(i, q) */
     __shared__ double sX[NB * Q];   /* x(c, q) */
-    __shared__ float sF[NB * SA];   /* R(k, j), j >= k */
-    __shared__ float sH[NB * Q];
+    __shared__ float sF[NB * SA];   /* R(k, j), j >= k; then R^-1(k, j) */
+    __shared__ float sV[NB * Q];    /* fl32(g)(k, q); then y(k, q) */
+    __shared__ float sD[NB * Q];    /* |h(k, q)| */
+    __shared__ double sN[NB * Q];   /* |x(k, q)| */
 
     const long kq = blockIdx.x;
     if (kq >= p.count) return;
@@ -116,26 +110,24 @@ __global__ __launch_bounds__(64) void mixed_refine_kernel(MixedRefineArgs p)
     const float *A32 = p.a32 + d.a_off;
     const float *B32 = p.b32 + d.b_off;
 
-    /* stage A (FP64) and R (FP32) */
-#pragma unroll 4
-    for (int c = 0; c < n; ++c) {
-        if (lane < m) sA[lane * SA + c] = A[(long long)c * d.lda + lane];
-        if (lane <= c) sF[lane * SA + c] = A32[(long long)c * d.lda + lane];
+    /* row i of A in registers (the residual), A and R in LDS */
+    double arow[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        arow[c] = (c < n && lane < m) ? A[(long long)c * d.lda + lane] : 0.0;
+        if (c < n && lane < m) sA[lane * SA + c] = arow[c];
+        if (c < n && lane <= c) sF[lane * SA + c] = A32[(long long)c * d.lda + lane];
     }
-    /* b in the residual layout (lane = row) */
     double bq[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q)
         bq[q] = (lane < m && q < nrhs) ? B[(long long)q * d.ldb + lane] : 0.0;
-    /* x_0 = the FP32 solution, in the solve layout: lane (rg, q), rows rg + G t */
-    const int qs = lane % Q, rg = lane / Q;
-    const bool qlive = qs < nrhs;
-    double xv[T];
+    /* x_0 = the FP32 solution, lane = row j < n */
+    double xq[Q];
 #pragma unroll
-    for (int t = 0; t < T; ++t) {
-        const int i = rg + G * t;
-        xv[t] = (i < n && qlive) ? (double)B32[(long long)qs * d.ldb + i] : 0.0;
-        if (i < NB) sX[i * Q + qs] = xv[t];
+    for (int q = 0; q < Q; ++q) {
+        xq[q] = (lane < n && q < nrhs) ? (double)B32[(long long)q * d.ldb + lane] : 0.0;
+        if (lane < NB) sX[lane * Q + q] = xq[q];
     }
     __syncthreads();
     bool ok = true;
@@ -143,149 +135,171 @@ __global__ __launch_bounds__(64) void mixed_refine_kernel(MixedRefineArgs p)
         const float r = sF[k * SA + k];
         ok = ok && r != 0.0f && __builtin_isfinite(r);
     }
+    if (!ok) goto fallback;
+    {
+        /* column j = lane of R^-1 by back substitution: z_j = 1 / R(j,j), then
+         * for i < j descending z_i = -(sum_{k=i+1..j} R(i,k) z_k) / R(i,i) */
+        float z[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+            z[k] = (k == lane && k < n) ? 1.0f / sF[k * SA + k] : 0.0f;
+#pragma unroll
+        for (int i = NB - 2; i >= 0; --i) {
+            if (i + 1 < n) {
+                float s = 0.0f;
+#pragma unroll
+                for (int k = i + 1; k < NB; ++k)
+                    if (k < n) s = __builtin_fmaf(sF[i * SA + k], z[k], s);
+                if (i < lane) z[i] = -s / sF[i * SA + i];
+            }
+        }
+        __syncthreads(); /* every lane has read R */
+        float rcol[NB], rrow[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            rcol[k] = z[k];
+            if (lane < NB) sF[k * SA + lane] = z[k]; /* R^-1(k, lane) */
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+            rrow[k] = (lane < n && k >= lane && k < n) ? sF[lane * SA + k] : 0.0f;
 
-    const double cte = 0x1p-53 * __builtin_sqrt((double)n), floor_ = 0x1p-40;
-    double prev = __builtin_inf();
-    bool done = !qlive; /* columns beyond nrhs take no part */
-    bool fail = false;
-    int steps = 0;
-    for (int it = 0; ok && it <= p.itermax; ++it) {
-        /* r = b - A x, lane = row */
-        if (lane < m) {
-            double r[Q];
+        const double cte = 0x1p-53 * __builtin_sqrt((double)n), floor_ = 0x1p-40;
+        /* lanes q < nrhs keep column q's state */
+        double prev = __builtin_inf();
+        bool done = lane >= nrhs, fail = false;
+        int steps = 0;
+        unsigned long long done_mask = 0; /* columns settled, bit q */
+        for (int it = 0; it <= p.itermax; ++it) {
+            /* r = b - A x, lane = row, columns ascending */
+            if (lane < m) {
+                double r[Q];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) r[q] = bq[q];
-            for (int c = 0; c < n; ++c) {
-                const double a = sA[lane * SA + c];
+                for (int q = 0; q < Q; ++q) r[q] = bq[q];
 #pragma unroll
-                for (int q = 0; q < Q; ++q) r[q] = __builtin_fma(-sX[c * Q + q], a, r[q]);
+                for (int c = 0; c < NB; ++c) {
+                    if (c < n) {
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) r[q] = __builtin_fma(-sX[c * Q + q], arow[c], r[q]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < Q; ++q) sR[lane * Q + q] = r[q];
             }
+            __syncthreads();
+            /* g = A^T r, lane = column, rows ascending */
+            if (lane < n) {
+                double g[Q];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) sR[lane * Q + q] = r[q];
-        }
-        __syncthreads();
-        /* g = A^T r, lane = column */
-        if (lane < n) {
-            double g[Q];
+                for (int q = 0; q < Q; ++q) g[q] = 0.0;
+#pragma unroll 8
+                for (int i = 0; i < m; ++i) {
+                    const double a = sA[i * SA + lane];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) g[q] = 0.0;
-            for (int i = 0; i < m; ++i) {
-                const double a = sA[i * SA + lane];
+                    for (int q = 0; q < Q; ++q) g[q] = __builtin_fma(a, sR[i * Q + q], g[q]);
+                }
 #pragma unroll
-                for (int q = 0; q < Q; ++q) g[q] = __builtin_fma(a, sR[i * Q + q], g[q]);
+                for (int q = 0; q < Q; ++q) sV[lane * Q + q] = (float)g[q];
             }
+            __syncthreads();
+            /* y = R^-T fl32(g): y_j = sum_{k <= j} R^-1(k, j) g_k, k ascending */
+            float y[Q];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) sH[lane * Q + q] = (float)g[q];
-        }
-        __syncthreads();
-        /* R^T R h = g in FP32, lane (rg, qs) holding rows rg + G t */
-        float hv[T];
+            for (int q = 0; q < Q; ++q) y[q] = 0.0f;
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-            const int i = rg + G * t;
-            hv[t] = i < n ? sH[i * Q + qs] : 0.0f;
-        }
-        /* forward, R^T y = g: y_k = y_k / R(k,k), then y_i -= R(k,i) y_k */
+            for (int k = 0; k < NB; ++k) {
+                if (k < n) {
 #pragma unroll
-        for (int kt = 0; kt < T; ++kt) {
-            for (int ko = 0; ko < G; ++ko) {
-                const int k = ko + G * kt;
-                if (k >= n) break;
-                if (rg == ko) hv[kt] = hv[kt] / sF[k * SA + k];
-                const float hk = shfl_f(hv[kt], ko * Q + qs);
-#pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    const int i = rg + G * t;
-                    if (i > k && i < n) hv[t] = __builtin_fmaf(-sF[k * SA + i], hk, hv[t]);
+                    for (int q = 0; q < Q; ++q) y[q] = __builtin_fmaf(rcol[k], sV[k * Q + q], y[q]);
                 }
             }
-        }
-        /* backward, R h = y: h_k = y_k / R(k,k), then y_i -= R(i,k) h_k, k descending */
+            __syncthreads(); /* every lane has read g */
 #pragma unroll
-        for (int kt = T - 1; kt >= 0; --kt) {
-            for (int ko = G - 1; ko >= 0; --ko) {
-                const int k = ko + G * kt;
-                if (k >= n) continue;
-                if (rg == ko) hv[kt] = hv[kt] / sF[k * SA + k];
-                const float hk = shfl_f(hv[kt], ko * Q + qs);
+            for (int q = 0; q < Q; ++q)
+                if (lane < NB) sV[lane * Q + q] = y[q];
+            __syncthreads();
+            /* h = R^-1 y: h_i = sum_{k >= i} R^-1(i, k) y_k, k ascending */
+            float h[Q];
 #pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    const int i = rg + G * t;
-                    if (i < k) hv[t] = __builtin_fmaf(-sF[i * SA + k], hk, hv[t]);
+            for (int q = 0; q < Q; ++q) h[q] = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                if (k < n) {
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) h[q] = __builtin_fmaf(rrow[k], sV[k * Q + q], h[q]);
                 }
             }
-        }
-        /* x += h for the live columns; ||h||_inf, ||x||_inf per column */
-        double dn = 0.0, xn = 0.0;
+            /* x += h on the open columns; |h|, |x| to LDS */
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-            const int i = rg + G * t;
-            if (i < n) {
-                dn = fmax_nan(dn, (double)__builtin_fabsf(hv[t]));
-                if (!done) xv[t] = xv[t] + (double)hv[t];
-                xn = fmax_nan(xn, __builtin_fabs(xv[t]));
+            for (int q = 0; q < Q; ++q) {
+                const bool open = ((done_mask >> q) & 1ull) == 0;
+                if (lane < n && open) xq[q] = xq[q] + (double)h[q];
+                if (lane < NB) {
+                    sD[lane * Q + q] = lane < n ? __builtin_fabsf(h[q]) : 0.0f;
+                    sN[lane * Q + q] = lane < n ? __builtin_fabs(xq[q]) : 0.0;
+                    sX[lane * Q + q] = xq[q];
+                }
             }
+            __syncthreads();
+            /* lane q decides column q (NaN-propagating maxima) */
+            if (lane < Q && !done) {
+                double dn = 0.0, xn = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    dn = fmax_nan(dn, (double)sD[k * Q + lane]);
+                    xn = fmax_nan(xn, sN[k * Q + lane]);
+                }
+                steps = it > steps ? it : steps;
+                if (dn <= xn * cte || (it > 0 && dn > 0.5 * prev && dn <= xn * floor_))
+                    done = true;
+                else if ((!(dn < prev) && it > 0) || it == p.itermax)
+                    fail = true; /* not contracting, NaN, or out of steps */
+                prev = dn;
+            }
+            done_mask = __builtin_amdgcn_ballot_w64(lane < Q && done);
+            if (__builtin_amdgcn_ballot_w64(lane < Q && fail) != 0) break;
+            if ((done_mask & ((1ull << Q) - 1ull)) == ((1ull << Q) - 1ull)) break;
         }
+        ok = (done_mask & ((1ull << Q) - 1ull)) == ((1ull << Q) - 1ull);
+        /* steps: the most any column took */
 #pragma unroll
-        for (int mask = Q; mask < 64; mask <<= 1) {
-            dn = fmax_nan(dn, shfl_xor_d(dn, mask));
-            xn = fmax_nan(xn, shfl_xor_d(xn, mask));
+        for (int mask = 1; mask < Q; mask <<= 1) {
+            const int o = __builtin_amdgcn_ds_bpermute(((lane ^ mask) & 63) << 2, steps);
+            steps = o > steps ? o : steps;
         }
-        if (!done) {
-            steps = it > steps ? it : steps;
-            if (dn <= xn * cte || (it > 0 && dn > 0.5 * prev && dn <= xn * floor_))
-                done = true;
-            else if ((!(dn < prev) && it > 0) || it == p.itermax)
-                fail = true; /* not contracting, NaN, or out of steps */
-            prev = dn;
-        }
-        /* every column settled, or one failed: the problem is decided */
-        if (__builtin_amdgcn_ballot_w64(fail) != 0) break;
-        if (__builtin_amdgcn_ballot_w64(!done) == 0) break;
+        if (ok) {
+            double *X = p.x + d.b_off;
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-            const int i = rg + G * t;
-            if (i < NB) sX[i * Q + qs] = xv[t];
+            for (int q = 0; q < Q; ++q)
+                if (lane < n && q < nrhs) X[(long long)q * d.ldb + lane] = xq[q];
+            if (lane == 0) {
+                p.iter[d.idx] = steps;
+                p.info[d.idx] = 0;
+            }
+            return;
         }
-        __syncthreads();
     }
-    ok = ok && __builtin_amdgcn_ballot_w64(fail || !done) == 0;
-    /* steps: the most any column took */
-#pragma unroll
-    for (int mask = 1; mask < 64; mask <<= 1) {
-        const int o = __builtin_amdgcn_ds_bpermute(((lane ^ mask) & 63) << 2, steps);
-        steps = o > steps ? o : steps;
-    }
-    double *X = p.x + d.b_off;
-    if (ok) {
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            const int i = rg + G * t;
-            if (i < n && qlive) X[(long long)qs * d.ldb + i] = xv[t];
-        }
-        if (lane == 0) {
-            p.iter[d.idx] = steps;
-            p.info[d.idx] = 0;
-        }
-        return;
-    }
+fallback:
     /* the FP64 path: X := B (m rows), then the problem joins its kernel's list */
-    for (int q = 0; q < nrhs; ++q)
-        if (lane < m) X[(long long)q * d.ldb + lane] = B[(long long)q * d.ldb + lane];
-    if (lane == 0) {
-        p.iter[d.idx] = -p.itermax - 1;
-        const unsigned slot = atomicAdd(p.fb_fill + d.k64, 1u);
-        BatchProblem e;
-        e.m = m;
-        e.n = n;
-        e.nrhs = nrhs;
-        e.lda = d.lda;
-        e.ldb = d.ldb;
-        e.idx = d.idx;
-        e.a_off = d.a_off;
-        e.b_off = d.b_off;
-        e.tau_off = 0;
-        p.fb[p.fb_begin[d.k64] + slot] = e;
+    {
+        double *X = p.x + d.b_off;
+        for (int q = 0; q < nrhs; ++q)
+            if (lane < m) X[(long long)q * d.ldb + lane] = B[(long long)q * d.ldb + lane];
+        if (lane == 0) {
+            p.iter[d.idx] = -p.itermax - 1;
+            const unsigned slot = atomicAdd(p.fb_fill + d.k64, 1u);
+            BatchProblem e;
+            e.m = m;
+            e.n = n;
+            e.nrhs = nrhs;
+            e.lda = d.lda;
+            e.ldb = d.ldb;
+            e.idx = d.idx;
+            e.a_off = d.a_off;
+            e.b_off = d.b_off;
+            e.tau_off = 0;
+            p.fb[p.fb_begin[d.k64] + slot] = e;
+        }
     }
 }
 
@@ -301,11 +315,10 @@ hipError_t launch_refine(const MixedRefineArgs &a, hipStream_t s)
 int mixed_refine_bucket(int m, int n, int nrhs)
 {
     const int mb = m <= 32 ? 0 : 1;
-    const int nb = n <= 16 ? 0 : n <= 32 ? 1 : 2;
+    const int nb = n <= 16 ? 0 : 1;
     const int qb = nrhs <= 4 ? 0 : 1;
-    /* (32,16), (32,32), (64,16), (64,32), (64,64), times Q in {4, 8} */
-    const int shape = mb == 0 ? nb : 2 + nb;
-    return shape * 2 + qb;
+    /* (32,16), (32,32), (64,16), (64,32), times Q in {4, 8} */
+    return (mb * 2 + nb) * 2 + qb;
 }
 
 hipError_t launch_mixed_refine(int bucket, const MixedRefineArgs &a, hipStream_t s)
@@ -321,8 +334,6 @@ hipError_t launch_mixed_refine(int bucket, const MixedRefineArgs &a, hipStream_t
     case 5: return launch_refine<64, 16, 8>(a, s);
     case 6: return launch_refine<64, 32, 4>(a, s);
     case 7: return launch_refine<64, 32, 8>(a, s);
-    case 8: return launch_refine<64, 64, 4>(a, s);
-    case 9: return launch_refine<64, 64, 8>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -332,13 +343,6 @@ hipError_t launch_mixed_prep(const MixedPrepArgs &a, hipStream_t s)
     if (a.count <= 0) return hipSuccess;
     hipLaunchKernelGGL(mixedbatch::mixed_prep_kernel, dim3((unsigned)((a.count + 3) / 4)), dim3(256), 0,
                        s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_init_counters(unsigned *dst, const unsigned *init, int n, hipStream_t s)
-{
-    hipLaunchKernelGGL(mixedbatch::init_counters_kernel, dim3((n + 255) / 256), dim3(256), 0, s, dst,
-                       init, n);
     return hipGetLastError();
 }
 

@@ -223,16 +223,18 @@ int sgels_batch(cbk_batch_plan_t plan, float *ap, float *bp, int *info, void *wo
                 size_t lwork);
 
 /* dsgels_batch: the batch in mixed precision (a plan made with prec 'M').
- * A problem with trans 'N', 1 <= n <= m <= 64 and 1 <= nrhs <= 8 is solved by
- * the FP32 batch gels (on FP32 copies of A and B in the workspace) and FP64
- * refinement of that solution on the seminormal equations, the per-column
- * scheme of dsgels_compact: r = B - A X and A^T r in FP64, R^T R D = A^T r
- * with the FP32 R, X += D, until ||D||_inf <= ||X||_inf eps sqrt(n) (eps =
- * 2^-53) or the correction stops halving once below ||X||_inf 2^-40. A
+ * A problem with trans 'N', 1 <= n <= m <= 64, n <= 32 and 1 <= nrhs <= 8 is
+ * solved by the FP32 batch gels (on FP32 copies of A and B in the workspace)
+ * and FP64 refinement of that solution on the seminormal equations, the
+ * per-column scheme of dsgels_compact: r = B - A X and A^T r in FP64,
+ * R^T R D = A^T r with the FP32 R (through R^-1, formed once), X += D, until
+ * ||D||_inf <= ||X||_inf eps sqrt(n) (eps = 2^-53) or the correction stops
+ * halving once below ||X||_inf 2^-40. A
  * problem that does not converge (a correction that does not contract, a
  * NaN, a zero or non-finite diagonal of the FP32 R, or 30 steps), and every
  * problem outside that range, is solved by the FP64 batch gels instead,
- * in place on A and X. Everything runs as one recorded graph.
+ * in place on A and X (those outside the range concurrently with the FP32
+ * work). Everything runs as one recorded graph.
  * A: read (left as it was for a refined problem; the FP64 QR for the rest,
  * as dgels_batch). B: read only. X (xp): laid out as B (the same b_off and
  * ldb, so a buffer the size of B); its first n rows of each problem receive

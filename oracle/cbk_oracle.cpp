@@ -936,6 +936,74 @@ bool refine_column(long m, long n, const LV<double> &A, const LV<double> &B, lon
     return false;
 }
 
+/* R^-1 of the upper triangle of F (n x n, float), column j by back
+ * substitution: z_j = 1 / R(j,j); for i = j-1 down to 0,
+ * z_i = -(sum_{k=i+1..j} R(i,k) z_k) / R(i,i), k ascending. Column-major,
+ * zeros below the diagonal. */
+std::vector<float> tri_inverse(long n, const LV<float> &F)
+{
+    std::vector<float> Ri((size_t)n * n, 0.0f);
+    for (long j = 0; j < n; ++j) {
+        float *z = &Ri[(size_t)j * n];
+        z[j] = 1.0f / F(j, j);
+        for (long i = j - 1; i >= 0; --i) {
+            float s = 0.0f;
+            for (long k = i + 1; k <= j; ++k)
+                s = std::fma(F(i, k), z[k], s);
+            z[i] = -s / F(i, i);
+        }
+    }
+    return Ri;
+}
+
+/* refine_column with the preconditioner applied through an explicit
+ * inverse: y = R^-T fl32(g) (y_j = sum_{k<=j} Ri(k,j) g_k, k ascending), then
+ * h = R^-1 y (h_i = sum_{k>=i} Ri(i,k) y_k, k ascending); the rest as
+ * refine_column. The library's dsgels_batch refinement. */
+bool refine_column_inv(long m, long n, const LV<double> &A, const LV<double> &B, long q,
+                       const std::vector<float> &Ri, std::vector<double> &x, int itermax,
+                       int &steps)
+{
+    const double cte = std::ldexp(1.0, -53) * std::sqrt((double)n), floor_ = std::ldexp(1.0, -40);
+    std::vector<double> gr(n);
+    std::vector<float> g(n), y(n);
+    double prev = INFINITY;
+    for (int it = 0; it <= itermax; ++it) {
+        for (long c = 0; c < n; ++c)
+            gr[c] = 0.0;
+        for (long i = 0; i < m; ++i) {
+            double r = B(i, q);
+            for (long c = 0; c < n; ++c)
+                r = std::fma(-x[c], A(i, c), r);
+            for (long c = 0; c < n; ++c)
+                gr[c] = std::fma(A(i, c), r, gr[c]);
+        }
+        for (long c = 0; c < n; ++c)
+            g[c] = (float)gr[c];
+        for (long j = 0; j < n; ++j) {
+            float s = 0.0f;
+            for (long k = 0; k <= j; ++k)
+                s = std::fma(Ri[(size_t)j * n + k], g[k], s);
+            y[j] = s;
+        }
+        double dn = 0.0, xn = 0.0;
+        for (long i = 0; i < n; ++i) {
+            float s = 0.0f;
+            for (long k = i; k < n; ++k)
+                s = std::fma(Ri[(size_t)k * n + i], y[k], s);
+            dn = nanmax(dn, std::fabs((double)s));
+            x[i] = x[i] + (double)s;
+        }
+        for (long i = 0; i < n; ++i)
+            xn = nanmax(xn, std::fabs(x[i]));
+        steps = it > steps ? it : steps;
+        if (dn <= xn * cte || (it > 0 && dn > 0.5 * prev && dn <= xn * floor_)) return true;
+        if ((!(dn < prev) && it > 0) || it == itermax) return false;
+        prev = dn;
+    }
+    return false;
+}
+
 } /* namespace */
 
 extern "C" {
@@ -1044,11 +1112,11 @@ int oracle_sgels_batch(char trans, int nprob, const oracle_gels_problem *desc, f
 }
 
 /* The mixed-precision uneven batch (the library's dsgels_batch; no reference
- * counterpart). A problem with trans 'N', 1 <= n <= m <= 64 and
+ * counterpart). A problem with trans 'N', 1 <= n <= m <= 64, n <= 32 and
  * 1 <= nrhs <= 8: the single precision gels (this file's gels_lane in float,
  * the reference's gels_compact_group, src/cbk_gels_compact.hpp:84-106) on
- * float copies of A and B gives R and x_0; each column is then refined as
- * dsgels_compact's (refine_column) from x_0. A problem that does not converge
+ * float copies of A and B gives R and x_0; each column is then refined from
+ * x_0 as dsgels_compact's, with R^-1 formed once (refine_column_inv). A problem that does not converge
  * (or has a zero or non-finite R(i,i)) gets iter = -31, every other problem
  * iter = -1, and both are solved by the double gels in place on A and on X
  * := B. X is laid out as B. */
@@ -1066,7 +1134,7 @@ int oracle_dsgels_batch(char trans, int nprob, const oracle_gels_problem *desc, 
         const LV<double> A = make_view(ap + d.a_off, false, d.lda, 1);
         const LV<double> B = make_view(const_cast<double *>(bp) + d.b_off, false, d.ldb, 1);
         const LV<double> X = make_view(xp + d.b_off, false, d.ldb, 1);
-        const bool mixed = !tran && n >= 1 && n <= m && m <= 64 && nrhs >= 1 && nrhs <= 8;
+        const bool mixed = !tran && n >= 1 && n <= m && m <= 64 && n <= 32 && nrhs >= 1 && nrhs <= 8;
         bool ok = mixed;
         int steps = 0;
         info[k] = 0;
@@ -1083,10 +1151,12 @@ int oracle_dsgels_batch(char trans, int nprob, const oracle_gels_problem *desc, 
             for (long i = 0; i < n && ok; ++i)
                 ok = Fv(i, i) != 0.f && std::isfinite(Fv(i, i));
             std::vector<std::vector<double>> xs(nrhs, std::vector<double>(n));
+            std::vector<float> Ri;
+            if (ok) Ri = tri_inverse(n, Fv);
             for (long q = 0; q < nrhs && ok; ++q) {
                 for (long i = 0; i < n; ++i)
                     xs[q][i] = (double)Sv(i, q);
-                ok = refine_column(m, n, A, B, q, Fv, xs[q], itermax, steps);
+                ok = refine_column_inv(m, n, A, B, q, Ri, xs[q], itermax, steps);
             }
             if (ok) {
                 for (long q = 0; q < nrhs; ++q)

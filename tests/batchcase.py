@@ -147,11 +147,12 @@ def mixed_mix(r, nuneven=200):
     returns (shapes, indices of the ill-conditioned, indices outside)"""
     shapes = [
         (16, 8, 2), (12, 5, 3), (32, 16, 5), (30, 30, 1),          # rows <= 32, n <= 16 / 32
-        (64, 8, 8), (50, 13, 4), (64, 32, 2), (40, 33, 7),         # rows <= 64, n <= 16 / 32 / 64
-        (64, 64, 8), (64, 50, 3), (4, 2, 1), (2, 1, 1),
-    ] + uneven_shapes(r, nuneven)
+        (64, 8, 8), (50, 13, 4), (64, 32, 2), (40, 31, 7),         # rows <= 64, n <= 16 / 32
+        (64, 17, 8), (4, 2, 1), (2, 1, 1),
+    ] + [s for s in uneven_shapes(r, nuneven) if s[1] <= 32]
     ill = list(range(len(shapes), len(shapes) + 4))
-    shapes += [(24, 12, 2), (64, 40, 3), (32, 32, 6), (48, 20, 8)]
-    outside = list(range(len(shapes), len(shapes) + 6))
-    shapes += [(70, 10, 2), (30, 40, 2), (20, 5, 9), (5, 0, 2), (6, 3, 0), (80, 64, 4)]
+    shapes += [(24, 12, 2), (64, 30, 3), (32, 32, 6), (48, 20, 8)]
+    outside = list(range(len(shapes), len(shapes) + 8))
+    shapes += [(70, 10, 2), (30, 40, 2), (20, 5, 9), (5, 0, 2), (6, 3, 0), (80, 64, 4),
+               (64, 33, 2), (40, 40, 8)]
     return shapes, ill, outside

@@ -124,13 +124,15 @@ def test_mixed_batch_k5_distribution_backward_error():
             if eta > worst:
                 worst, at = eta, (k, d, eta, int(itg[k]))
     assert worst <= 1e-15, at
-    assert np.mean(itg >= 0) > 0.99, np.unique(itg, return_counts=True)
+    inside = np.array([d[1] <= 32 for d in desc])
+    assert np.all(itg[~inside] == -1)
+    assert np.mean(itg[inside] >= 0) > 0.99, np.unique(itg, return_counts=True)
 
 
 def test_mixed_batch_new_buffers_rerecord_and_transposed():
     torch = _t()
     r = cbkt.rng(4202)
-    shapes = [(16, 8, 2), (40, 33, 5), (64, 64, 8), (70, 10, 2)]
+    shapes = [(16, 8, 2), (40, 31, 5), (64, 20, 8), (70, 10, 2)]
     desc, ap, bp, As, Bs = batchcase.make(r, shapes)
     print("case dsgels_batch re-record", flush=True)
     plan = cbk_amd.GelsBatchPlan("M", "N", desc)

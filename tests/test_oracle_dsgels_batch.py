@@ -6,9 +6,9 @@ library's dsgels_batch; no reference counterpart):
   - cond 1e12 problems: iter = -31, and A and X bit-identical to the oracle's
     plain double batch gels (pinned against LAPACK and the reference in
     test_oracle_batch.py) on the same problem, X := B first;
-  - shapes outside the single precision path (more than 64 rows,
-    underdetermined, nrhs > 8, n = 0 or nrhs = 0) and every transposed
-    problem: iter = -1 and the plain double result, bit for bit;
+  - shapes outside the single precision path (more than 64 rows, more than
+    32 columns, underdetermined, nrhs > 8, n = 0 or nrhs = 0) and every
+    transposed problem: iter = -1 and the plain double result, bit for bit;
   - the refinement itself is the one dsgels_compact uses (refine_column),
     which test_oracle_lapack.py pins against LAPACK on its own.
 """
