@@ -35,7 +35,12 @@ Also reported, in the same JSON line:
                MI355X's 8 TB/s; traffic from the rocprofv3 counters recorded in
                profiles/);
   cpu_baseline the reference's own CPU path (oracle/_ref, the reference's
-               src/cbk.cpp compiled from its sources) timed on this host.
+               src/cbk.cpp compiled from its sources) timed on this host in
+               a child process with OMP_PROC_BIND=close OMP_PLACES=cores, on
+               every physical core this process may use and on 1 thread:
+               dposv n=32 nrhs=4 (the headline shape), K1 exactly as
+               configured (dposv n=8 nrhs=1, 65536 matrices, V=8) and dgels
+               64x32 nrhs=2; lscpu's sockets x cores x threads recorded.
 """
 
 import argparse
@@ -63,6 +68,7 @@ def parse():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch gels leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the K1-K3 configuration leg")
     ap.add_argument("--uneven-nprob", type=int, default=K5_NPROB)
+    ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (the known-size kernel "
                          "that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -131,47 +137,140 @@ def load_traffic():
         return json.load(f)
 
 
-def cpu_baseline(A, B):
-    """the reference's dposv_compact (oracle/_ref) on this host's cores, FP64,
-    n=32, nrhs=4, V=8 (the AVX-512 width), on the benchmark's own matrices;
-    best of 3 passes with an untimed restore (the reference's best_time,
-    examples/bench_util.hpp:250-267)"""
+def host_topology():
+    """sockets, cores per socket and threads per core (lscpu), and the CPUs
+    this process may run on"""
+    import subprocess
+    topo = {"sockets": None, "cores_per_socket": None, "threads_per_core": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                topo[keys[k.strip()]] = int(v.strip())
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                topo["model"] = line.partition(":")[2].strip()
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    topo["affinity_cpus"] = len(os.sched_getaffinity(0))
+    return topo
+
+
+def cpu_leg_main():
+    """Child process of the CPU baseline (OpenMP pinned by the environment the
+    parent sets): times the reference's own kernels (oracle/_ref, its
+    src/cbk.cpp compiled from its sources; the oracle's restatement if that
+    build is absent) on bounded samples of the workloads and prints one JSON
+    line. Best of 3 after a warm-up, inputs restored outside the timed calls
+    (the reference's best_time, examples/bench_util.hpp:250-267)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import cbkt
-    nm = A.shape[0]
     path = cbkt.ref_path()
-    kind = "reference"
-    if path is None:
-        lib, prefix, kind = ctypes.CDLL(cbkt.ORACLE_SO), "oracle_", "port"
-    else:
-        lib, prefix = ctypes.CDLL(path), ""
-    fn = getattr(lib, prefix + "dposv_compact")
-    fn.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
-                   ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-    fn.restype = ctypes.c_int
-    Vc = 8
-    ap0 = cbkt.pack(A, Vc, fill=0.0)
-    bp0 = cbkt.pack(B, Vc, pad_diag=0.0, fill=0.0)
+    kind = "reference" if path else "port"
+    lib, prefix = (ctypes.CDLL(path), "") if path else (ctypes.CDLL(cbkt.ORACLE_SO), "oracle_")
+    name = os.path.basename(path) if path else "oracle port"
+    posv = getattr(lib, prefix + "dposv_compact")
+    posv.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                     ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    posv.restype = ctypes.c_int
+    gels = getattr(lib, prefix + "dgels_compact")
+    gels.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                     ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                     ctypes.c_int, ctypes.c_int]
+    gels.restype = ctypes.c_int
+    r = np.random.default_rng(2030)
+
+    def best_of(call, restore):
+        best = float("inf")
+        for rep in range(4):
+            restore()
+            t0 = time.perf_counter()
+            assert call() == 0
+            dt = time.perf_counter() - t0
+            if rep > 0:
+                best = min(best, dt)
+        return best
+
+    def spd(nm, n, kmax=(2, 4, 6)):
+        G = r.standard_normal((nm, n, n))
+        Q, Rr = np.linalg.qr(G)
+        lam = 10.0 ** (r.uniform(0, 1, (nm, n)) * np.array(kmax)[r.integers(0, len(kmax), nm)][:, None])
+        return (Q * lam[:, None, :]) @ np.swapaxes(Q, 1, 2)
+
+    out = {}
+    # K4 shape: dposv n=32 nrhs=4 FP64, V=8, 131072 matrices (the headline's distribution)
+    nm, n, q, Vc = 131072, 32, 4, 8
+    A = spd(nm, n)
+    B = r.standard_normal((nm, n, q))
+    ap0, bp0 = cbkt.pack(A, Vc, fill=0.0), cbkt.pack(B, Vc, pad_diag=0.0, fill=0.0)
     ap, bp = ap0.copy(), bp0.copy()
-    best = float("inf")
-    for rep in range(4):  # one warm-up, then best of 3
+
+    def restore():
         ap[...] = ap0
         bp[...] = bp0
-        t0 = time.perf_counter()
-        st = fn(b"C", b"L", N, NRHS, ap.ctypes.data, N, bp.ctypes.data, N, Vc, nm)
-        dt = time.perf_counter() - t0
-        assert st == 0
-        if rep > 0:
-            best = min(best, dt)
-    X = cbkt.unpack(bp, nm, N, NRHS, Vc)
-    Rs = B - A @ X
-    eta = np.max(np.abs(Rs).max(axis=1) / (np.abs(A).sum(axis=2).max(axis=1)[:, None] *
-                                            np.abs(X).max(axis=1) + np.abs(B).max(axis=1)))
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": nm / best, "unit": "matrices/s", "cores": cores, "kind": kind,
-            "sample": f"dposv_compact n=32 nrhs=4 FP64 V=8, {nm} matrices, best of 3, "
-                      f"{os.path.basename(path) if path else 'oracle port'}; max backward error {eta:.1e}"}
+    t = best_of(lambda: posv(b"C", b"L", n, q, ap.ctypes.data, n, bp.ctypes.data, n, Vc, nm), restore)
+    X = cbkt.unpack(bp, nm, n, q, Vc)
+    eta = float(np.max(np.abs(B - A @ X).max(axis=1) / (np.abs(A).sum(axis=2).max(axis=1)[:, None] *
+                                                        np.abs(X).max(axis=1) + np.abs(B).max(axis=1))))
+    out["K4_dposv_n32"] = {"value": nm / t, "unit": "matrices/s", "seconds": t,
+                           "sample": f"dposv_compact n=32 nrhs=4 FP64 V=8, {nm} matrices, {name}; "
+                                     f"max backward error {eta:.1e}"}
+    del A, B, ap0, bp0, ap, bp
+    # K1 exactly as configured: dposv n=8 nrhs=1, 65536 matrices, V=8
+    nm, n, q = 65536, 8, 1
+    G = r.standard_normal((nm, n, n))
+    A = G @ np.swapaxes(G, 1, 2) + 8 * np.eye(n)
+    B = r.standard_normal((nm, n, q))
+    ap0, bp0 = cbkt.pack(A, Vc, fill=0.0), cbkt.pack(B, Vc, pad_diag=0.0, fill=0.0)
+    ap, bp = ap0.copy(), bp0.copy()
+    t = best_of(lambda: posv(b"C", b"L", n, q, ap.ctypes.data, n, bp.ctypes.data, n, Vc, nm), restore)
+    out["K1_dposv_n8"] = {"value": nm / t, "unit": "matrices/s", "seconds": t,
+                          "sample": f"dposv_compact n=8 nrhs=1 FP64 V=8, {nm} matrices, {name}"}
+    # dgels n=32: 64 x 32, nrhs=2, U(-1,1), 131072 problems, V=8
+    nm, m, n, q = 131072, 64, 32, 2
+    A = r.uniform(-1, 1, (nm, m, n))
+    B = r.uniform(-1, 1, (nm, m, q))
+    ap0, bp0 = cbkt.pack(A, Vc, fill=0.0), cbkt.pack(B, Vc, pad_diag=0.0, fill=0.0)
+    ap, bp = ap0.copy(), bp0.copy()
+    tau = np.zeros(((nm + Vc - 1) // Vc) * Vc * n)
+    t = best_of(lambda: gels(b"C", b"N", m, n, q, ap.ctypes.data, m, bp.ctypes.data, m,
+                             tau.ctypes.data, Vc, nm), restore)
+    out["dgels_64x32"] = {"value": nm / t, "unit": "matrices/s", "seconds": t,
+                          "sample": f"dgels_compact 64x32 nrhs=2 FP64 V=8, {nm} problems, {name}"}
+    print(json.dumps({"kind": kind, "legs": out}), flush=True)
+
+
+def cpu_baseline():
+    """the reference's CPU path on this host: a child process per thread
+    count (all the physical cores this process may use, then 1), with
+    OMP_PROC_BIND=close and OMP_PLACES=cores"""
+    import subprocess
+    topo = host_topology()
+    tpc = topo.get("threads_per_core") or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    phys = max(1, topo["affinity_cpus"] // tpc)
+    threads = min(phys, share) if share > 0 else phys
+    runs = {}
+    for nt in (threads, 1):
+        env = dict(os.environ, OMP_NUM_THREADS=str(nt), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg"], env=env,
+                           capture_output=True, text=True, timeout=900)
+        if p.returncode != 0:
+            raise RuntimeError(f"CPU leg failed ({p.returncode}): {p.stderr[-2000:]}")
+        runs[nt] = json.loads(p.stdout.strip().splitlines()[-1])
+    full, one = runs[threads], runs[1]
+    head = full["legs"]["K4_dposv_n32"]
+    legs = {}
+    for k, v in full["legs"].items():
+        legs[k] = dict(v, threads=threads)
+        legs[k + "_1thread"] = dict(one["legs"][k], threads=1)
+    return {"value": head["value"], "unit": "matrices/s", "cores": threads, "kind": full["kind"],
+            "sample": head["sample"] + f"; {threads} threads, OMP_PROC_BIND=close OMP_PLACES=cores",
+            "host": dict(topo, omp_proc_bind="close", omp_places="cores"), "legs": legs}
 
 
 GELS_M, GELS_N, GELS_NRHS, GELS_NM = 64, 32, 2, 131072
@@ -577,6 +676,9 @@ def uneven_bench(torch, K, W, nprob=K5_NPROB):
 
 def main():
     args = parse()
+    if args.cpu_leg:
+        cpu_leg_main()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}). "
               "Matrices are independent, so N GPUs would each run this kernel on nm/N matrices.",
@@ -714,7 +816,7 @@ def main():
         result["uneven_gels"] = uneven_bench(torch, K=max(2, min(K, 5)), W=1, nprob=args.uneven_nprob)
     if not args.no_cpu:
         log("CPU baseline (reference dposv_compact on the host) ...")
-        result["cpu_baseline"] = cpu_baseline(A_host, B_host)
+        result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)
 
 
