@@ -429,6 +429,11 @@ def configs_bench(torch):
         t = timeit(lambda: cbk_amd.call(p, "trsm", "C", "L", "L", "N", "U", n, q, 1.0, A0, n, B, n, V, nm),
                    lambda: B.copy_(B0))
         rec(f"K3_{p}trsm_24x24", t, nm, es * (n * (n - 1) // 2 + 2 * n * q), f"{p}trsm_compact L L N U 24x24 nrhs=24")
+        # side R: X A = B with A upper (the side-L kernels on B's transposed view)
+        t = timeit(lambda: cbk_amd.call(p, "trsm", "C", "R", "U", "N", "U", n, q, 1.0, A0, n, B, n, V, nm),
+                   lambda: B.copy_(B0))
+        rec(f"K3_{p}trsm_R_24x24", t, nm, es * (n * (n - 1) // 2 + 2 * n * q),
+            f"{p}trsm_compact R U N U 24x24, 24 rows of B")
         del A0, B0, A, B
     torch.cuda.empty_cache()
 

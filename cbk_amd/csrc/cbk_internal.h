@@ -96,11 +96,13 @@ hipError_t launch_sym_team(bool ldl, bool factor, bool up, int n, int nrhs, Oper
                            Operand<T> B, int V, int nm, int *info);
 
 /* trsm, side L, with one lane per matrix and op(A)'s triangle in registers
- * (trsm_reg.hip): m <= 16, and m <= 24 in FP32 */
+ * (trsm_reg.hip): m <= 16, and m <= 24 in FP32. strided: every column by row
+ * dots, never the axpy form the reference's side-L router takes for the last
+ * of an odd column count (the side-R solves run through here transposed) */
 bool trsm_reg_supports(bool left, int m, bool fp64);
 template <class T>
 hipError_t launch_trsm_reg(bool up, bool tran, bool unit, int m, int n, T alpha, Operand<T> A,
-                           Operand<T> B, int V, int nm);
+                           Operand<T> B, int V, int nm, bool strided = false);
 
 /* trsm, side L, forward sweep (op(A) lower) with one matrix per 4 lanes
  * (quad_trsm.hip): FP64 m = 24, FP32 m = 32 */

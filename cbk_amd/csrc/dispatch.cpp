@@ -29,6 +29,24 @@ hipError_t dispatch_trsm(bool left, bool up, bool tran, bool unit, int m, int n,
 {
     const int s = left ? m : n;
     const bool fit = offsets_fit(A, s, s, V) && offsets_fit(B, m, n, V);
+    const bool fp64 = sizeof(T) == 8;
+    if (!left && fit && current_policy() != CBK_POLICY_LANE) {
+        /* X op(A) = alpha B is op(A)^T X^T = alpha B^T: the side-L kernels on
+         * B's transposed view (the same buffer with its strides swapped) with
+         * trans flipped. Per element of X this is the reference's side-R
+         * sequence (src/cbk_trsm_compact.hpp:206-226): the multiply by alpha,
+         * the fma's over l ascending, then the divide -- the side-L kernels'
+         * row-dot sequence (never the axpy form of the side-L router, whose
+         * back sweep subtracts in descending order) -- so the result is
+         * bit-identical. */
+        const Operand<T> Bt{B.p, B.ld, !B.row, B.gstride};
+        if (trsm_reg_supports(true, n, fp64))
+            return launch_trsm_reg<T>(up, !tran, unit, n, m, alpha, A, Bt, V, nm, true);
+        if (quad_trsm_supports(true, up, !tran, n, fp64))
+            return launch_quad_trsm<T>(!tran, unit, n, m, alpha, A, Bt, V, nm);
+        if (V >= 4 && trsm_team_supports(true, up, !tran, n))
+            return launch_trsm_team<T>(up, !tran, unit, n, m, alpha, A, Bt, V, nm);
+    }
     if (fit && current_policy() != CBK_POLICY_LANE && trsm_reg_supports(left, m, sizeof(T) == 8))
         return launch_trsm_reg<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
     if (fit && current_policy() != CBK_POLICY_LANE && quad_trsm_supports(left, up, tran, m, sizeof(T) == 8))

@@ -165,7 +165,7 @@ bool trsm_reg_supports(bool left, int m, bool fp64)
 
 template <class T>
 hipError_t launch_trsm_reg(bool up, bool tran, bool unit, int m, int n, T alpha, Operand<T> A,
-                           Operand<T> B, int V, int nm)
+                           Operand<T> B, int V, int nm, bool strided)
 {
     RegTrsmArgs<T> a;
     const int asi = A.row ? A.ld : 1, asj = A.row ? 1 : A.ld;
@@ -186,7 +186,7 @@ hipError_t launch_trsm_reg(bool up, bool tran, bool unit, int m, int n, T alpha,
      * form for the last of an odd column count when the operand it sees is
      * not transposed */
     const bool odd = (n % 4 == 1) || (n % 4 == 3);
-    a.axpy_last = odd && a.b_si == 1 && (asi == 1 ? !tran : (asj == 1 && tran));
+    a.axpy_last = !strided && odd && a.b_si == 1 && (asi == 1 ? !tran : (asj == 1 && tran));
     a.V = V;
     a.nlanes = (long)((nm + V - 1) / V) * V;
     const bool back = (up != tran);
@@ -199,8 +199,8 @@ hipError_t launch_trsm_reg(bool up, bool tran, bool unit, int m, int n, T alpha,
 }
 
 template hipError_t launch_trsm_reg<double>(bool, bool, bool, int, int, double, Operand<double>,
-                                            Operand<double>, int, int);
+                                            Operand<double>, int, int, bool);
 template hipError_t launch_trsm_reg<float>(bool, bool, bool, int, int, float, Operand<float>,
-                                           Operand<float>, int, int);
+                                           Operand<float>, int, int, bool);
 
 } /* namespace cbk */
