@@ -265,7 +265,8 @@ constexpr int kItermax = 30; /* LAPACK dsposv's ITERMAX */
  * 32-bit offsets fit (offsets_fit) */
 bool mixed_uses_team(bool row, int n, int nrhs, int ldap, int ldbp, int ldxp, int V)
 {
-    return current_policy() == CBK_POLICY_AUTO && V >= 4 && mixed_team_supports(n) &&
+    return current_policy() == CBK_POLICY_AUTO && V >= 4 &&
+           (mixed_team_supports(n) || quad_mixed_supports(n)) &&
            offsets_fit_bytes(gstride(row, ldap, n, n, V), ldap, row, n, n, V, 8) &&
            offsets_fit_bytes(gstride(row, ldbp, n, nrhs, V), ldbp, row, n, nrhs, V, 8) &&
            offsets_fit_bytes(gstride(row, ldxp, n, nrhs, V), ldxp, row, n, nrhs, V, 8);
@@ -308,9 +309,13 @@ int mixed_sv(bool ldl, char layout, char uplo, int n, int nrhs, double *ap, int 
     const Operand<double> A = operand(ap, ldap, row, n, n, V);
     const Operand<double> B = operand(const_cast<double *>(bp), ldbp, row, n, nrhs, V);
     const Operand<double> X = operand(xp, ldxp, row, n, nrhs, V);
-    if (mixed_uses_team(row, n, nrhs, ldap, ldbp, ldxp, V))
+    if (mixed_uses_team(row, n, nrhs, ldap, ldbp, ldxp, V)) {
+        if (quad_mixed_supports(n))
+            return status(
+                launch_mixed_posv_quad(ldl, up, n, nrhs, A, B, X, V, nm, kItermax, iter, info));
         return status(
             launch_mixed_posv_team(ldl, up, n, nrhs, A, B, X, V, nm, kItermax, iter, info));
+    }
     return status(launch_mixed_posv_lane(ldl, up, n, nrhs, A, B, X, static_cast<float *>(work),
                                          V, nm, kItermax, iter, info));
 }

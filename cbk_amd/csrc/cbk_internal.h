@@ -81,11 +81,12 @@ hipError_t launch_reg_chol(bool ldl, bool factor, bool up, int n, int nrhs, Oper
 
 /* orders served by the potrf/posv/sytrfnp/sysvnp quad kernel (one matrix per
  * 4 lanes, quad_sym.hip): FP64 24 and 32, FP32 32; any V; same contract as
- * launch_chol_lane */
+ * launch_chol_lane. only: if set, the call touches only the matrices with
+ * only[mat] < 0 (their A, B and info; the others are left as they are). */
 bool quad_sym_supports(int n, bool fp64);
 template <class T>
 hipError_t launch_quad_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
-                           Operand<T> B, int V, int nm, int *info);
+                           Operand<T> B, int V, int nm, int *info, const int *only = nullptr);
 
 /* orders served by the potrf/posv/sytrfnp/sysvnp team kernel */
 bool sym_team_supports(int n);
@@ -133,6 +134,13 @@ hipError_t launch_ormqr_team(bool left, bool tran, int m, int n, int k, Operand<
 /* orders served by the fused mixed posv / sysvnp team kernel (any nrhs, in
  * blocks of 4 right-hand sides) */
 bool mixed_team_supports(int n);
+
+/* orders served by the fused mixed posv / sysvnp quad kernel (quad_mixed.hip,
+ * one matrix per 4 lanes), followed by the masked FP64 quad pass */
+bool quad_mixed_supports(int n);
+hipError_t launch_mixed_posv_quad(bool ldl, bool up, int n, int nrhs, Operand<double> A,
+                                  Operand<double> B, Operand<double> X, int V, int nm,
+                                  int itermax, int *iter, int *info);
 
 /* mixed posv (ldl: sysvnp), team path (no workspace); A is the operand as
  * passed (uplo via up) */

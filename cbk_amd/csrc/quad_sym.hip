@@ -15,9 +15,10 @@ bool quad_sym_supports(int n, bool fp64) { return fp64 ? (n == 24 || n == 32) : 
 
 template <class T>
 hipError_t launch_quad_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
-                           Operand<T> B, int V, int nm, int *info)
+                           Operand<T> B, int V, int nm, int *info, const int *only)
 {
     quadsym::QuadSymArgs<T> args;
+    args.only = only;
     args.a = A.p;
     /* the lower view: uplo 'U' reads A transposed (src/cbk_common.hpp:201-206) */
     int si = A.row ? A.ld : 1, sj = A.row ? 1 : A.ld;
@@ -50,8 +51,8 @@ hipError_t launch_quad_sym(bool ldl, bool factor, bool up, int n, int nrhs, Oper
 }
 
 template hipError_t launch_quad_sym<double>(bool, bool, bool, int, int, Operand<double>,
-                                            Operand<double>, int, int, int *);
+                                            Operand<double>, int, int, int *, const int *);
 template hipError_t launch_quad_sym<float>(bool, bool, bool, int, int, Operand<float>,
-                                           Operand<float>, int, int, int *);
+                                           Operand<float>, int, int, int *, const int *);
 
 } /* namespace cbk */

@@ -40,6 +40,7 @@ template <class T> struct QuadSymArgs {
     bool factor; /* false: potrs / sytrsnp on a given factor */
     int q0;      /* first right-hand side of this launch (one-block instances) */
     const int *skip; /* one-block launches after the first: the info the first wrote */
+    const int *only; /* if set, only the matrices with only[mat] < 0 are touched */
 };
 
 /* resident wavefronts per SIMD the kernel is built for: the FP64 order 32
@@ -72,6 +73,12 @@ quad_sym_kernel(QuadSymArgs<T> p)
 
     QuadPlace pl;
     if (!quad_place(p.nmat, p.V, pl)) return; /* whole wavefront */
+    if (p.only) {
+        /* a masked pass: the other matrices' slots read as zero and their
+         * stores drop; a wavefront with none to do leaves at once */
+        pl.live = pl.live && pl.mat < p.nm && p.only[pl.mat] < 0;
+        if (__builtin_amdgcn_ballot_w64(pl.live) == 0) return;
+    }
     const int t = pl.t;
     const unsigned VS = (unsigned)p.V * SZ;
     const __amdgpu_buffer_rsrc_t ra =
@@ -136,7 +143,7 @@ quad_sym_kernel(QuadSymArgs<T> p)
             });
         };
         linfo = quad_factor_panels<T, N, LDL>(a, dg, t, load, store);
-        if (p.info && pl.mat < p.nm && t == 0) p.info[pl.mat] = linfo;
+        if (p.info && pl.live && pl.mat < p.nm && t == 0) p.info[pl.mat] = linfo;
         if constexpr (!prefetch_b<T, N>()) {
             if (p.nrhs > 0) load_b(b, qfirst);
         }
