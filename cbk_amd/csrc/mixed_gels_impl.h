@@ -45,7 +45,8 @@ namespace mixedgels {
 using namespace team;
 
 struct MixedGelsArgs {
-    const double *a; /* p x q, element (i, c) at (i*a_si + c*a_sj)*V in the group */
+    int m, n; /* the matrix: m <= P rows, n <= Q columns of the kernel's bucket */
+    const double *a; /* m x n, element (i, c) at (i*a_si + c*a_sj)*V in the group */
     int a_si, a_sj;
     size_t a_gs;
     const double *b; /* p x nrhs */
@@ -78,11 +79,18 @@ template <class T> __device__ __forceinline__ T team_sum(T x)
     return x;
 }
 
-template <int P, int Q, int NRB>
+/* resident wavefronts per SIMD: one for the large buckets, and whenever
+ * four right-hand sides go through the loop together */
+template <int P, int Q, int NRB> constexpr int gels_waves() { return (P * Q >= 1024 || NRB > 2) ? 1 : 2; }
+
+/* EXACT: the matrix is P x Q (m and n fold away); otherwise m <= P rows and
+ * n <= Q columns at run time */
+template <int P, int Q, int NRB, bool EXACT>
 __global__ __launch_bounds__(teams_per_block<P>() * TS)
-__attribute__((amdgpu_waves_per_eu(P >= 64 || NRB > 2 ? 1 : 2, P >= 64 || NRB > 2 ? 1 : 8))) void
+__attribute__((amdgpu_waves_per_eu(gels_waves<P, Q, NRB>(), gels_waves<P, Q, NRB>() == 1 ? 1 : 8))) void
 mixed_gels_kernel(MixedGelsArgs p)
 {
+    const int m = EXACT ? P : p.m, n = EXACT ? Q : p.n;
     constexpr int R = P / TS;  /* row blocks of A */
     constexpr int RQ = (Q + TS - 1) / TS; /* row blocks of R */
     constexpr unsigned SZ = 8;
@@ -101,16 +109,20 @@ mixed_gels_kernel(MixedGelsArgs p)
     const __amdgpu_buffer_rsrc_t rx = group_rsrc(p.x + g * p.x_gs, group_range(groups_left, p.x_gs, SZ));
     const unsigned a_col = (unsigned)p.a_sj * V * SZ, b_col = (unsigned)p.b_sj * V * SZ,
                    x_col = (unsigned)p.x_sj * V * SZ;
+    /* rows past m read as zero (they change neither R nor the residuals);
+     * columns past n read as zero and get R(c,c) = 1, which decouples them */
     unsigned a_row[R], b_row[R], x_row[RQ];
     sfor<0, R>([&](auto r) {
         const int i = t + TS * r;
-        a_row[r] = ((unsigned)(i * p.a_si) * V + v) * SZ;
-        b_row[r] = ((unsigned)(i * p.b_si) * V + v) * SZ;
+        a_row[r] = (i < m) ? ((unsigned)(i * p.a_si) * V + v) * SZ : kOOB;
+        b_row[r] = (i < m) ? ((unsigned)(i * p.b_si) * V + v) * SZ : kOOB;
     });
     sfor<0, RQ>([&](auto r) {
         const int i = t + TS * r;
-        x_row[r] = (i < Q) ? ((unsigned)(i * p.x_si) * V + v) * SZ : kOOB;
+        x_row[r] = (i < n) ? ((unsigned)(i * p.x_si) * V + v) * SZ : kOOB;
     });
+    bool colin[Q];
+    sfor<0, Q>([&](auto c) { colin[c] = c < n; });
 
     /* ---- FP32 QR of A (geqr2, qr_team_impl.h), R only; the load of A also
      * forms the first step's g = A^T b of the first block of right-hand
@@ -132,7 +144,7 @@ mixed_gels_kernel(MixedGelsArgs p)
                 });
             }
             sfor<0, Q>([&](auto c) {
-                const double a = bload(0.0, ra, a_row[r], c * a_col);
+                const double a = bload(0.0, ra, colin[c] ? a_row[r] : kOOB, c * a_col);
                 f[r][c] = (float)a;
                 if constexpr (FOLD) {
                     sfor<0, NRB>([&](auto j) { gp0[j][c] = __builtin_fma(a, bv[j], gp0[j][c]); });
@@ -155,7 +167,7 @@ mixed_gels_kernel(MixedGelsArgs p)
             const bool has = tail > 0.f;
             const float tau = has ? (beta - x0) / beta : 0.f;
             const float inv = has ? 1.f / (x0 - beta) : 0.f;
-            const float rdiag = has ? beta : x0;
+            const float rdiag = (kk < n) ? (has ? beta : x0) : 1.f;
             float vi[R];
             sfor<rk, R>([&](auto r) {
                 const int i = t + TS * r;
@@ -184,7 +196,7 @@ mixed_gels_kernel(MixedGelsArgs p)
     }
 
     int steps = 0;
-    const double cte = 0x1p-53 * __builtin_sqrt((double)Q), floor_ = 0x1p-40;
+    const double cte = 0x1p-53 * __builtin_sqrt((double)n), floor_ = 0x1p-40;
     /* NRB right-hand sides in lockstep: one pass over A per step serves them all */
     for (int q0 = 0; q0 < p.nrhs && ok; q0 += NRB) {
         double x[NRB][RQ], prev[NRB];
@@ -208,7 +220,7 @@ mixed_gels_kernel(MixedGelsArgs p)
             sfor<0, NRB>([&](auto j) { dpp_ready_from<0>(x[j]); });
             if (!first) sfor<0, R>([&](auto r) {
                 double arow[Q];
-                sfor<0, Q>([&](auto c) { arow[c] = bload(0.0, ra, a_row[r], c * a_col); });
+                sfor<0, Q>([&](auto c) { arow[c] = bload(0.0, ra, colin[c] ? a_row[r] : kOOB, c * a_col); });
                 sfor<0, NRB>([&](auto j) {
                     double res = bload(0.0, rb, b_row[r], bcol[j]);
                     sfor<0, Q>([&](auto c) { fnma_bcr<c % TS>(res, x[j][c / TS], arow[c]); });
@@ -264,16 +276,16 @@ mixed_gels_kernel(MixedGelsArgs p)
     if (mat < p.nm && t == 0) p.iter[mat] = ok ? steps : -p.itermax - 1;
 }
 
-template <int P, int Q> hipError_t launch_pq(const MixedGelsArgs &args)
+template <int P, int Q, bool EXACT> hipError_t launch_pq(const MixedGelsArgs &args)
 {
     constexpr int TPB = teams_per_block<P>();
     unsigned blocks = (unsigned)((args.nmat + TPB - 1) / TPB);
     if (P >= 64) blocks = xcd_grid(blocks);
     if (args.nrhs <= 2)
-        hipLaunchKernelGGL((mixed_gels_kernel<P, Q, 2>), dim3(blocks), dim3(TPB * TS), 0,
+        hipLaunchKernelGGL((mixed_gels_kernel<P, Q, 2, EXACT>), dim3(blocks), dim3(TPB * TS), 0,
                            current_stream(), args);
     else
-        hipLaunchKernelGGL((mixed_gels_kernel<P, Q, 4>), dim3(blocks), dim3(TPB * TS), 0,
+        hipLaunchKernelGGL((mixed_gels_kernel<P, Q, 4, EXACT>), dim3(blocks), dim3(TPB * TS), 0,
                            current_stream(), args);
     return hipGetLastError();
 }

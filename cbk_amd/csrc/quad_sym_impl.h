@@ -62,7 +62,10 @@ template <class T, int N> constexpr bool loops_rhs() { return waves_per_simd<T, 
  * SIMD, has no registers to spare for it */
 template <class T, int N> constexpr bool prefetch_b() { return !(sizeof(T) == 8 && N == 24); }
 
-template <class T, int N, int NRB, bool LDL>
+/* MASKED: the instance that the masked passes after the mixed-precision
+ * kernel launch (QuadSymArgs::only set), a kernel of its own so that traces
+ * keep the two apart; either instance honours only */
+template <class T, int N, int NRB, bool LDL, bool MASKED = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_simd<T, N>(),
                                                                      waves_per_simd<T, N>()))) void
 quad_sym_kernel(QuadSymArgs<T> p)
@@ -184,7 +187,7 @@ quad_sym_kernel(QuadSymArgs<T> p)
     }
 }
 
-template <class T, int N, int NRB, bool LDL> hipError_t launch_nrb(const QuadSymArgs<T> &args)
+template <class T, int N, int NRB, bool LDL, bool MASKED> hipError_t launch_nrb_m(const QuadSymArgs<T> &args)
 {
     const long waves = (args.nmat + kMatsPerWave - 1) / kMatsPerWave;
     const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
@@ -192,7 +195,7 @@ template <class T, int N, int NRB, bool LDL> hipError_t launch_nrb(const QuadSym
         QuadSymArgs<T> a1 = args;
         a1.q0 = 0;
         a1.skip = nullptr;
-        hipLaunchKernelGGL((quad_sym_kernel<T, N, NRB, LDL>), grid, block, 0, current_stream(), a1);
+        hipLaunchKernelGGL((quad_sym_kernel<T, N, NRB, LDL, MASKED>), grid, block, 0, current_stream(), a1);
         return hipGetLastError();
     }
     /* one launch per block of NRB right-hand sides; the factor from the first */
@@ -202,11 +205,21 @@ template <class T, int N, int NRB, bool LDL> hipError_t launch_nrb(const QuadSym
         a1.factor = args.factor && q0 == 0;
         a1.skip = (q0 > 0 && args.factor) ? args.info : nullptr;
         if (q0 > 0) a1.info = nullptr;
-        hipLaunchKernelGGL((quad_sym_kernel<T, N, NRB, LDL>), grid, block, 0, current_stream(), a1);
+        hipLaunchKernelGGL((quad_sym_kernel<T, N, NRB, LDL, MASKED>), grid, block, 0, current_stream(), a1);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+template <class T, int N, int NRB, bool LDL> hipError_t launch_nrb(const QuadSymArgs<T> &args)
+{
+    /* the masked passes follow the mixed-precision kernels, which the FP64
+     * order 24 serves */
+    if constexpr (sizeof(T) == 8 && N == 24) {
+        if (args.only) return launch_nrb_m<T, N, NRB, LDL, true>(args);
+    }
+    return launch_nrb_m<T, N, NRB, LDL, false>(args);
 }
 
 /* one order: NRB = 4 right-hand sides per sweep (2 for the FP64 order-24

@@ -149,8 +149,9 @@ size_t cbk_noalloc_scratch_bytes(void);
  * or until the correction stops halving once below ||x||_inf * 2^-40 (the
  * rounding floor of A^T r for an inconsistent system). Per matrix:
  *     iter  >= 0 refinement steps (the most any column took; A untouched);
- *           -1 the shape has no FP32 kernel (m x n not 16 x 8, 32 x 16 or
- *           64 x 32, nrhs > 8, or V < 4) and -31 refinement failed (a zero or
+ *           -1 the shape has no FP32 kernel (n > 32, or V < 4; the FP32 QR
+ *           runs in the smallest of the 16/32/64-row by 8/16/32-column
+ *           buckets that holds m x n) and -31 refinement failed (a zero or
  *           non-finite diagonal of the FP32 R, a correction that does not
  *           contract, or 30 steps): the FP64 gels ran and A holds its QR
  *           (reflectors below the diagonal, as gels_compact);

@@ -1018,9 +1018,8 @@ extern "C" {
  * contract, or 30 steps send the matrix to the double gels (A then holds its
  * QR, X the first n rows of the solution; info as gels_compact_info). iter:
  * refinement steps (max over the columns), -1 no single precision path for
- * the shape (m x n not 16 x 8, 32 x 16 or 64 x 32, nrhs > 8, V < 4), -31 the
- * double path ran. Shapes: trans 'N', n <= m <= 64,
- * n + nrhs <= 64. */
+ * the shape (n > 32, or V < 4), -31 the double path ran. Shapes: trans 'N',
+ * n <= m <= 64, n + nrhs <= 64. */
 int oracle_dsgels_compact(char layout, char trans, int m, int n, int nrhs, double *ap, int ldap,
                           const double *bp, int ldbp, double *xp, int ldxp, int *iter,
                           int *info, int V, int nm)
@@ -1041,8 +1040,7 @@ int oracle_dsgels_compact(char layout, char trans, int m, int n, int nrhs, doubl
         if (info) info[i] = 0;
     }
     if (n == 0 || nrhs == 0 || nm == 0) return 0;
-    const bool mixed = V >= 4 && nrhs <= 8 &&
-                       ((m == 16 && n == 8) || (m == 32 && n == 16) || (m == 64 && n == 32));
+    const bool mixed = V >= 4 && n >= 1 && n <= 32;
     const size_t sa = group_stride(row, ldap, m, n, V), sb = group_stride(row, ldbp, m, nrhs, V),
                  sx = group_stride(row, ldxp, n, nrhs, V);
     const int itermax = 30;

@@ -149,7 +149,7 @@ def test_dsgels_oracle_against_lapack():
     the FP64 path (iter -31, A = its QR) for kappa ~ 1e6, iter -1 for a shape
     without a single-precision kernel; info through the FP64 path"""
     r = cbkt.rng(12)
-    for (m, n, nrhs, V) in ((64, 32, 2, 8), (16, 8, 3, 4), (24, 10, 2, 4)):
+    for (m, n, nrhs, V) in ((64, 32, 2, 8), (16, 8, 3, 4), (24, 10, 2, 4), (40, 36, 2, 4)):
         nm = 10
         A = r.uniform(-1, 1, (nm, m, n))
         A[[2, 7]] = _tall_cond(r, 2, m, n, 6)
@@ -164,7 +164,7 @@ def test_dsgels_oracle_against_lapack():
         assert np.array_equal(bp, bp0)
         X = cbkt.unpack(xp, nm, n, nrhs, V)
         F = cbkt.unpack(ap, nm, m, n, V)
-        mixed = (m, n) in ((16, 8), (32, 16), (64, 32))
+        mixed = n <= 32
         for k in range(nm):
             _, x, inf = lapack.dgels(A[k], B[k])[:3]
             assert inf == 0 and info[k] == 0
