@@ -105,6 +105,31 @@ def call_ext(name, *args):
     return f(*[_ptr(a) if k == "p" else abi.to_c(k, a) for k, a in zip(kinds, args)])
 
 
+def call_compat(scalar, routine, *args):
+    """Call cbk_{scalar}{routine}_compact of include/cbk_compat.h (the
+    MKL-style adapter) with device tensors or addresses for its buffers and None
+    in the info slot; returns the host info it wrote (None for trsm, which has
+    none)."""
+    key = ("compat", scalar, routine)
+    if key not in _fns:
+        kinds = abi.COMPAT[routine]
+        _fns[key] = (abi.bind(lib(), f"cbk_{scalar}{routine}_compact", kinds, scalar, None), kinds)
+    f, kinds = _fns[key]
+    if len(args) != len(kinds):
+        raise TypeError(f"cbk_{scalar}{routine}_compact: expected {len(kinds)} arguments, got {len(args)}")
+    info = ctypes.c_int(-99)
+    cargs = []
+    for k, a in zip(kinds, args):
+        if k == "h":
+            cargs.append(ctypes.pointer(info))
+        elif k == "p":
+            cargs.append(_ptr(a))
+        else:
+            cargs.append(abi.to_c(k, a))
+    f(*cargs)
+    return info.value if "h" in kinds else None
+
+
 def dsposv_worksize(layout, uplo, n, nrhs, ldap, ldbp, ldxp, V, nm):
     f = lib().dsposv_compact_worksize
     f.argtypes = [ctypes.c_char, ctypes.c_char] + [ctypes.c_int] * 7

@@ -37,6 +37,25 @@ for _p in "ds":
     EXT[f"cbk_{_p}gepack_compact"] = "ciipipiii"
     EXT[f"cbk_{_p}geunpack_compact"] = "ciipipiii"
 
+# the MKL-style adapter of include/cbk_compat.h (void; 'h' = the host int* info)
+COMPAT = {
+    "geqrf": "iiipippihii",
+    "ormqr": "icciiipippipihii",
+    "orgqr": "iiiipippihii",
+    "potrf": "iiipihii",
+    "sytrfnp": "iiipihii",
+    "potrs": "iiiipipihii",
+    "posv": "iiiipipihii",
+    "sytrsnp": "iiiipipihii",
+    "sysvnp": "iiiipipihii",
+    "trsm": "iiiiiii" + "T" + "pipiii",
+    "gels": "iciiipipipihii",
+}
+# MKL's enum values (include/cbk_compat.h)
+MKL = {"ROW_MAJOR": 101, "COL_MAJOR": 102, "NOTRANS": 111, "TRANS": 112, "CONJTRANS": 113,
+       "UPPER": 121, "LOWER": 122, "NONUNIT": 131, "UNIT": 132, "LEFT": 141, "RIGHT": 142,
+       "COMPACT_SSE": 181, "COMPACT_AVX": 182, "COMPACT_AVX512": 183, "COMPACT_V64": 190}
+
 # per-matrix info variants: the plain argument list plus a trailing device int[nm]
 INFO_ROUTINES = ("potrf", "posv", "sytrfnp", "sysvnp", "gels")
 
@@ -50,6 +69,8 @@ def ctype_of(kind, scalar):
         return ctypes.c_void_p
     if kind == "z":
         return ctypes.c_size_t
+    if kind == "h":
+        return ctypes.POINTER(ctypes.c_int)
     if kind == "T":
         return ctypes.c_double if scalar == "d" else ctypes.c_float
     raise ValueError(kind)
