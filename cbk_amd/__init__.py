@@ -291,6 +291,46 @@ def autotune():
     return _bind_batch().cbk_autotune()
 
 
+FAMILY = {"default": 0, "register": 1, "quad": 2, "team": 3, "columns": 4, "wave": 5, "lane": 6}
+
+
+def _bind_tune():
+    L = lib()
+    if not getattr(L, "_tune_bound", False):
+        key = [ctypes.c_char, ctypes.c_char_p, ctypes.c_char, ctypes.c_char] + [ctypes.c_int] * 4
+        L.cbk_autotune_compact.argtypes = key + [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int)]
+        L.cbk_autotune_compact.restype = ctypes.c_int
+        L.cbk_set_kernel_family.argtypes = key + [ctypes.c_int]
+        L.cbk_set_kernel_family.restype = ctypes.c_int
+        L.cbk_get_kernel_family.argtypes = key
+        L.cbk_get_kernel_family.restype = ctypes.c_int
+        L._tune_bound = True
+    return L
+
+
+def _key(prec, routine, uplo, trans, m, n, nrhs, V):
+    return (abi.to_c("c", prec), routine.encode(), abi.to_c("c", uplo), abi.to_c("c", trans),
+            m, n, nrhs, V)
+
+
+def autotune_compact(prec, routine, m, n, nrhs, V, nm=0, uplo="L", trans="N"):
+    """cbk_autotune_compact: (status, family installed, V chosen)"""
+    f, v = ctypes.c_int(-1), ctypes.c_int(-1)
+    st = _bind_tune().cbk_autotune_compact(*_key(prec, routine, uplo, trans, m, n, nrhs, V), nm,
+                                           ctypes.byref(f), ctypes.byref(v))
+    return st, f.value, v.value
+
+
+def set_kernel_family(prec, routine, m, n, nrhs, V, family, uplo="L", trans="N"):
+    return _bind_tune().cbk_set_kernel_family(*_key(prec, routine, uplo, trans, m, n, nrhs, V),
+                                              family)
+
+
+def get_kernel_family(prec, routine, m, n, nrhs, V, uplo="L", trans="N"):
+    return _bind_tune().cbk_get_kernel_family(*_key(prec, routine, uplo, trans, m, n, nrhs, V))
+
+
 def get_tuning():
     t = Tuning()
     assert _bind_batch().cbk_get_tuning(ctypes.byref(t)) == 0

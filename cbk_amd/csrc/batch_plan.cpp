@@ -623,14 +623,17 @@ int dsgels_batch_impl(cbk_batch_plan *pl, double *ap, const double *bp, double *
 /* ---- the tuner ------------------------------------------------------------- */
 
 /* time one class kernel over `count` copies of an m x n problem with nrhs
- * right-hand sides (uniform(-1,1) data), best of 3, in milliseconds */
+ * right-hand sides (uniform(-1,1) data), best of 3, in milliseconds; per_cu:
+ * the persistent grid's blocks per compute unit (0: the kernel's occupancy) */
 template <class T>
-hipError_t time_class(int cls, int m, int n, int nrhs, int count, int dev, float *ms)
+hipError_t time_class(int cls, int m, int n, int nrhs, int count, int dev, float *ms,
+                      int per_cu = 0)
 {
     cbk_batch_plan pl;
     pl.prec = sizeof(T) == 8 ? 'D' : 'S';
     pl.device = dev;
     pl.tune = default_tuning();
+    pl.tune.blocks_per_cu[class_of_kernel(cls)] = per_cu;
     pl.nprob = count;
     pl.table.resize(count);
     const long long sa = (long long)m * n, sb = (long long)m * nrhs;
@@ -820,7 +823,9 @@ int cbk_set_tuning(const cbk_tuning *t)
  * 12, 16} (n = w - 2, nrhs = 2; m = the bucket), time the team kernel and the
  * wavefront kernel on 16384 identical problems; narrow_max_cols[b] is the
  * largest w up to which the team kernel is not slower at every w tried.
- * Grid sizes stay at each kernel's occupancy. Runs in FP64. */
+ * Then per class, at a shape typical of it, the persistent grid at half,
+ * once and twice the kernel's occupancy per compute unit; blocks_per_cu[c]
+ * keeps the fastest (0 when the occupancy itself is). Runs in FP64. */
 int cbk_autotune(void)
 {
     int dev = 0;
@@ -841,6 +846,31 @@ int cbk_autotune(void)
             maxw = w;
         }
         t.narrow_max_cols[b] = maxw;
+    }
+    /* grid sizes: class, the kernel behind it at the shape, m, n, nrhs, count */
+    const int shapes[6][6] = {{0, 0, 16, 6, 2, 16384},  {1, 1, 32, 6, 2, 16384},
+                              {2, 2, 64, 6, 2, 16384},  {3, 7, 32, 20, 4, 16384},
+                              {4, 8, 64, 24, 4, 16384}, {5, 5, 64, 60, 6, 4096}};
+    for (const auto &sh : shapes) {
+        int occ = 0;
+        e = cbk::gels_batch_occupancy<double>(sh[1], &occ);
+        if (e != hipSuccess) return cbk::launch_status(e);
+        occ = occ < 1 ? 1 : occ;
+        const int tries[3] = {occ, occ > 1 ? occ / 2 : 1, 2 * occ};
+        float best = 0.f;
+        int pick = 0;
+        for (int i = 0; i < 3; ++i) {
+            if (i == 1 && tries[1] == occ) continue;
+            float ms = 0.f;
+            e = time_class<double>(sh[1], sh[2], sh[3], sh[4], sh[5], dev, &ms, tries[i]);
+            if (e != hipSuccess) return cbk::launch_status(e);
+            /* another grid must win by more than 2 % to replace the occupancy */
+            if (i == 0 || ms < 0.98f * best) {
+                best = ms;
+                pick = i == 0 ? 0 : tries[i];
+            }
+        }
+        t.blocks_per_cu[sh[0]] = pick > 64 ? 64 : pick;
     }
     return cbk_set_tuning(&t) == 0 ? 0 : -1;
 }

@@ -70,6 +70,15 @@ hipError_t launch_mixed_posv_lane(bool ldl, bool up, int n, int nrhs, Operand<do
                                   Operand<double> B, Operand<double> X, float *work, int V,
                                   int nm, int itermax, int *iter, int *info);
 
+/* ---- kernel families chosen per shape (tune_compact.cpp) ----
+ * The dispatcher's built-in order can be overridden per shape by a family
+ * (CBK_FAMILY_* of cbk_ext.h) that cbk_autotune_compact measured fastest or
+ * cbk_set_kernel_family installed; 0 when the table has no entry. Keys: the
+ * routine class, the precision, the dimensions, nrhs and V. */
+enum TuneClass { kTuneChol = 0, kTuneLdl = 1, kTuneTrsmFwd = 2, kTuneTrsmBwd = 3, kTuneGels = 4,
+                 kTuneGeqrf = 5 };
+int tuned_family(int cls, bool fp64, int p, int q, int nrhs, int V);
+
 /* ---- team kernels: one matrix per 16-lane team, held in registers ---- */
 
 /* orders n <= 16 in FP64, n <= 24 in FP32: the register lane kernel

@@ -12,11 +12,31 @@ hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand
 {
     /* 32-bit offsets of the register and team kernels (offsets_fit) */
     const bool fit = offsets_fit(A, n, n, V) && offsets_fit(B, n, nrhs, V);
+    const bool fp64 = sizeof(T) == 8;
+    /* a family the tuner chose for this shape, where it serves the call */
+    if (fit && current_policy() == CBK_POLICY_AUTO) {
+        switch (tuned_family(ldl ? kTuneLdl : kTuneChol, fp64, n, 0, nrhs, V)) {
+        case CBK_FAMILY_REGISTER:
+            if (reg_chol_supports(n, fp64))
+                return launch_reg_chol<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+            break;
+        case CBK_FAMILY_QUAD:
+            if (quad_sym_supports(n, fp64))
+                return launch_quad_sym<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+            break;
+        case CBK_FAMILY_TEAM:
+            if (sym_team_supports(n) && V >= 4)
+                return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+            break;
+        case CBK_FAMILY_LANE: return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+        default: break;
+        }
+    }
     /* the whole triangle in one lane's registers where it fits (measured
      * faster than the team kernel at its orders 16 and 24 too) */
-    if (fit && current_policy() != CBK_POLICY_LANE && reg_chol_supports(n, sizeof(T) == 8))
+    if (fit && current_policy() != CBK_POLICY_LANE && reg_chol_supports(n, fp64))
         return launch_reg_chol<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
-    if (fit && current_policy() == CBK_POLICY_AUTO && quad_sym_supports(n, sizeof(T) == 8))
+    if (fit && current_policy() == CBK_POLICY_AUTO && quad_sym_supports(n, fp64))
         return launch_quad_sym<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     if (fit && current_policy() == CBK_POLICY_AUTO && sym_team_supports(n) && V >= 4)
         return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
@@ -30,6 +50,26 @@ hipError_t dispatch_trsm(bool left, bool up, bool tran, bool unit, int m, int n,
     const int s = left ? m : n;
     const bool fit = offsets_fit(A, s, s, V) && offsets_fit(B, m, n, V);
     const bool fp64 = sizeof(T) == 8;
+    if (left && fit && current_policy() == CBK_POLICY_AUTO) {
+        /* op(A) lower (up == tran) is the forward sweep */
+        switch (tuned_family(up == tran ? kTuneTrsmFwd : kTuneTrsmBwd, fp64, m, n, 0, V)) {
+        case CBK_FAMILY_REGISTER:
+            if (trsm_reg_supports(left, m, fp64))
+                return launch_trsm_reg<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
+            break;
+        case CBK_FAMILY_QUAD:
+            if (quad_trsm_supports(left, up, tran, m, fp64))
+                return launch_quad_trsm<T>(tran, unit, m, n, alpha, A, B, V, nm);
+            break;
+        case CBK_FAMILY_TEAM:
+            if (V >= 4 && trsm_team_supports(left, up, tran, m))
+                return launch_trsm_team<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
+            break;
+        case CBK_FAMILY_LANE:
+            return launch_trsm_lane<T>(left, up, tran, unit, m, n, alpha, A, B, V, nm);
+        default: break;
+        }
+    }
     if (!left && fit && current_policy() != CBK_POLICY_LANE) {
         /* X op(A) = alpha B is op(A)^T X^T = alpha B^T: the side-L kernels on
          * B's transposed view (the same buffer with its strides swapped) with
@@ -64,13 +104,36 @@ hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operan
     const int p = tall ? m : n, q = tall ? n : m;
     const bool overdet = (tall != tran) && offsets_fit(A, m, n, V) &&
                          offsets_fit(B, p, nrhs, V); /* src/cbk_gels_compact.hpp:124 */
+    const bool fp64 = sizeof(T) == 8;
+    if (overdet && current_policy() == CBK_POLICY_AUTO) {
+        switch (tuned_family(kTuneGels, fp64, p, q, nrhs, V)) {
+        case CBK_FAMILY_REGISTER:
+            if (qr_reg_supports(p, q, fp64))
+                return launch_qr_reg<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
+            break;
+        case CBK_FAMILY_TEAM:
+            if (V >= 4 && qr_team_supports(p, q, nrhs, fp64))
+                return launch_qr_team<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
+            break;
+        case CBK_FAMILY_COLUMNS:
+            if (V >= 4 && qr_cols_supports(p, q, nrhs))
+                return launch_qr_cols<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
+            break;
+        case CBK_FAMILY_WAVE:
+            if (qr_wave_supports(p, q, nrhs))
+                return launch_qr_wave<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
+            break;
+        case CBK_FAMILY_LANE: return launch_gels_lane<T>(tran, m, n, nrhs, A, B, tau, V, nm, info);
+        default: break;
+        }
+    }
     /* F in one lane's registers where it fits; then the row team at the
      * orders it is tuned for (its butterfly sums measured faster there), the
      * column team for the other small shapes */
-    if (current_policy() != CBK_POLICY_LANE && overdet && qr_reg_supports(p, q, sizeof(T) == 8))
+    if (current_policy() != CBK_POLICY_LANE && overdet && qr_reg_supports(p, q, fp64))
         return launch_qr_reg<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
     if (current_policy() == CBK_POLICY_AUTO && overdet && V >= 4 &&
-        qr_team_supports(p, q, nrhs, sizeof(T) == 8))
+        qr_team_supports(p, q, nrhs, fp64))
         return launch_qr_team<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
     if (current_policy() == CBK_POLICY_AUTO && overdet && V >= 4 && qr_cols_supports(p, q, nrhs))
         return launch_qr_cols<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
@@ -83,10 +146,29 @@ template <class T> hipError_t dispatch_geqrf(int m, int n, Operand<T> A, T *tau,
 {
     const Operand<T> none = {nullptr, 1, A.row, 0};
     const bool tall = m >= n && offsets_fit(A, m, n, V);
-    if (current_policy() != CBK_POLICY_LANE && tall && qr_reg_supports(m, n, sizeof(T) == 8))
+    const bool fp64 = sizeof(T) == 8;
+    if (tall && current_policy() == CBK_POLICY_AUTO) {
+        switch (tuned_family(kTuneGeqrf, fp64, m, n, 0, V)) {
+        case CBK_FAMILY_REGISTER:
+            if (qr_reg_supports(m, n, fp64))
+                return launch_qr_reg<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
+            break;
+        case CBK_FAMILY_TEAM:
+            if (V >= 4 && qr_team_supports(m, n, 0, fp64))
+                return launch_qr_team<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
+            break;
+        case CBK_FAMILY_COLUMNS:
+            if (V >= 4 && qr_cols_supports(m, n, 0))
+                return launch_qr_cols<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
+            break;
+        case CBK_FAMILY_LANE: return launch_geqrf_lane<T>(m, n, A, tau, V, nm);
+        default: break;
+        }
+    }
+    if (current_policy() != CBK_POLICY_LANE && tall && qr_reg_supports(m, n, fp64))
         return launch_qr_reg<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
     if (current_policy() == CBK_POLICY_AUTO && tall && V >= 4 &&
-        qr_team_supports(m, n, 0, sizeof(T) == 8))
+        qr_team_supports(m, n, 0, fp64))
         return launch_qr_team<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
     if (current_policy() == CBK_POLICY_AUTO && tall && V >= 4 && qr_cols_supports(m, n, 0))
         return launch_qr_cols<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);

@@ -259,8 +259,11 @@ int cbk_batch_plan_destroy(cbk_batch_plan_t plan);
  * wavefront kernel (one problem per 64 lanes) by the largest column count
  * narrow_max_cols[b] the team kernel should take, and size each persistent
  * grid to blocks_per_cu[c] blocks of 256 threads per compute unit.
- * cbk_autotune measures both on the current device and stream with synthetic
- * problems (allocating and freeing its own device memory) and installs the
+ * cbk_autotune measures both on the current device with synthetic problems
+ * (allocating and freeing its own device memory), then times each class's
+ * kernel at a typical shape with its persistent grid at half, once and twice
+ * its occupancy per compute unit (blocks_per_cu: the fastest, 0 for the
+ * occupancy itself unless another is more than 2 % faster), and installs the
  * result for plans created afterwards on this device. */
 typedef struct cbk_tuning {
     int narrow_max_cols[3];
@@ -270,6 +273,46 @@ typedef struct cbk_tuning {
 int cbk_autotune(void);
 int cbk_get_tuning(cbk_tuning *t);
 int cbk_set_tuning(const cbk_tuning *t);
+
+/* ---- kernel families of the compact routines -----------------------------
+ * A compact routine runs each shape on one of several kernel families, which
+ * differ in how many lanes one matrix gets (how its tile is laid over the
+ * wavefront). The dispatcher's built-in order was measured on MI355X; the
+ * tuner below measures the families that can serve one shape on the current
+ * device and installs the fastest for it (per device; the table is consulted
+ * under CBK_POLICY_AUTO, before the built-in order). */
+#define CBK_FAMILY_DEFAULT 0  /* the built-in order (no table entry) */
+#define CBK_FAMILY_REGISTER 1 /* one matrix per lane, held in registers */
+#define CBK_FAMILY_QUAD 2     /* one matrix per 4 lanes */
+#define CBK_FAMILY_TEAM 3     /* one matrix per 16 lanes, rows in lanes */
+#define CBK_FAMILY_COLUMNS 4  /* one matrix per 16 lanes, columns in lanes (QR) */
+#define CBK_FAMILY_WAVE 5     /* one matrix per wavefront (QR) */
+#define CBK_FAMILY_LANE 6     /* the general one-lane-per-matrix kernel */
+
+/* Measure and install the fastest family for one shape. routine: "posv"
+ * (also potrf / potrs, whose nrhs is 0 / the call's), "sysvnp" (sytrfnp /
+ * sytrsnp), "trsm" (side L: A m x m with uplo and trans, B m x n), "gels"
+ * (trans 'N', m >= n, nrhs) or "geqrf" (m >= n). The routine runs nm matrices
+ * (0: a count that fills the device, at most 2^17) of synthetic data
+ * (diagonally dominant symmetric, well-conditioned triangular, or uniform
+ * (-1,1)), column-major with ld = rows, best of 3 timed calls per family.
+ * V: the interleave width to tune at; 0 tunes each of 16, 32 and 64 and
+ * reports in *vbest the one with the most matrices per second. *family
+ * receives the family installed (for *vbest when V = 0). Allocates and frees
+ * its own device memory; synchronous. Codes: -1 prec, -2 routine, -3 uplo or
+ * trans, -5 m, -6 n, -7 nrhs, -8 V, -9 nm, CBK_ERR_LAUNCH. */
+int cbk_autotune_compact(char prec, const char *routine, char uplo, char trans, int m, int n,
+                         int nrhs, int V, int nm, int *family, int *vbest);
+
+/* Install (family > 0) or remove (CBK_FAMILY_DEFAULT) the family of one
+ * shape, same key as cbk_autotune_compact; a family that cannot serve the
+ * shape is ignored at dispatch. Codes as cbk_autotune_compact, -10 family. */
+int cbk_set_kernel_family(char prec, const char *routine, char uplo, char trans, int m, int n,
+                          int nrhs, int V, int family);
+/* the family installed for the shape, CBK_FAMILY_DEFAULT if none (or a
+ * negative code as above) */
+int cbk_get_kernel_family(char prec, const char *routine, char uplo, char trans, int m, int n,
+                          int nrhs, int V);
 
 /* ---- dense <-> compact ------------------------------------------------------
  * Conversion on the device between nm dense matrices (rows x cols) and the

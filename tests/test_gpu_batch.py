@@ -163,6 +163,7 @@ def test_batch_tuning_routes_and_autotune():
         assert cbk_amd.autotune() == 0
         t = cbk_amd.get_tuning()
         assert all(0 <= x <= 16 for x in t["narrow_max_cols"])
+        assert all(0 <= x <= 64 for x in t["blocks_per_cu"])
         print("tuned", t, flush=True)
     finally:
         cbk_amd.set_tuning(saved["narrow_max_cols"], saved["blocks_per_cu"])
