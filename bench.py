@@ -499,6 +499,35 @@ def ls_backward_error(desc, Ah, Bh, Xpool, sample):
     return worst
 
 
+def compact_tuner_bench():
+    """cbk_autotune_compact at the configurations' shapes (2^17 synthetic
+    matrices each): the kernel family it installs, and at K4's shape also the
+    interleave width it picks among 16, 32 and 64. Runs after every timed leg
+    (the families it installs would otherwise steer them)."""
+    import cbk_amd
+    names = {v: k for k, v in cbk_amd.FAMILY.items()}
+    shapes = [("K1_dposv_n8", "d", "posv", 8, 8, 1, "L", "N"),
+              ("K2_dgels_16x8", "d", "gels", 16, 8, 2, "L", "N"),
+              ("K2_dgeqrf_16x8", "d", "geqrf", 16, 8, 0, "L", "N"),
+              ("K3_dsysvnp_n24", "d", "sysvnp", 24, 24, 24, "L", "N"),
+              ("K3_ssysvnp_n24", "s", "sysvnp", 24, 24, 24, "L", "N"),
+              ("K3_dtrsm_24x24", "d", "trsm", 24, 24, 0, "L", "N"),
+              ("K3_strsm_24x24", "s", "trsm", 24, 24, 0, "L", "N"),
+              ("K4_dposv_n32", "d", "posv", 32, 32, 4, "L", "N")]
+    out = {}
+    for name, p, routine, m, n, nrhs, uplo, trans in shapes:
+        t0 = time.perf_counter()
+        st, fam, v = cbk_amd.autotune_compact(p, routine, m, n, nrhs, 64, uplo=uplo, trans=trans)
+        out[name] = {"status": st, "family": names.get(fam, fam), "V": v,
+                     "seconds": time.perf_counter() - t0}
+    t0 = time.perf_counter()
+    st, fam, v = cbk_amd.autotune_compact("d", "posv", 32, 32, 4, 0)
+    out["K4_dposv_n32_pick_V"] = {"status": st, "family": names.get(fam, fam), "V": v,
+                                  "seconds": time.perf_counter() - t0}
+    log(f"compact tuner: {out}")
+    return out
+
+
 def uneven_bench(torch, K, W, nprob=K5_NPROB):
     """dgels_batch / sgels_batch over the uneven batch (one graph launch per
     step, inputs re-copied outside the timed region), against the same problems
@@ -818,7 +847,18 @@ def main():
     if not args.no_configs:
         result["configs"] = configs_bench(torch)
     if not args.no_uneven:
+        # the uneven batch's class thresholds and grid sizes from the tuner
+        log("batch tuner (cbk_autotune) ...")
+        t0 = time.perf_counter()
+        st = cbk_amd.autotune()
+        tuned = cbk_amd.get_tuning()
+        tuned["seconds"] = time.perf_counter() - t0
+        tuned["status"] = st
+        log(f"batch tuning: {tuned}")
         result["uneven_gels"] = uneven_bench(torch, K=max(2, min(K, 5)), W=1, nprob=args.uneven_nprob)
+        result["uneven_gels"]["tuning"] = tuned
+    if not args.no_configs:
+        result["compact_tuner"] = compact_tuner_bench()
     if not args.no_cpu:
         log("CPU baseline (reference dposv_compact on the host) ...")
         result["cpu_baseline"] = cpu_baseline()
