@@ -68,6 +68,10 @@ def parse():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch gels leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the K1-K3 configuration leg")
     ap.add_argument("--uneven-nprob", type=int, default=K5_NPROB)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the tuners (their launches share kernel names and grids with the "
+                         "timed legs, which a profile has to keep apart); the uneven leg then "
+                         "plans with the default tuning")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (the known-size kernel "
@@ -499,11 +503,15 @@ def ls_backward_error(desc, Ah, Bh, Xpool, sample):
     return worst
 
 
+TUNE_NM = 98304
+
+
 def compact_tuner_bench():
-    """cbk_autotune_compact at the configurations' shapes (2^17 synthetic
-    matrices each): the kernel family it installs, and at K4's shape also the
-    interleave width it picks among 16, 32 and 64. Runs after every timed leg
-    (the families it installs would otherwise steer them)."""
+    """cbk_autotune_compact at the configurations' shapes (98304 synthetic
+    matrices each, a count no timed leg uses, so that profiles keep the
+    tuner's dispatches apart): the kernel family it installs, and at K4's
+    shape also the interleave width it picks among 16, 32 and 64. Runs after
+    every timed leg (the families it installs would otherwise steer them)."""
     import cbk_amd
     names = {v: k for k, v in cbk_amd.FAMILY.items()}
     shapes = [("K1_dposv_n8", "d", "posv", 8, 8, 1, "L", "N"),
@@ -517,11 +525,12 @@ def compact_tuner_bench():
     out = {}
     for name, p, routine, m, n, nrhs, uplo, trans in shapes:
         t0 = time.perf_counter()
-        st, fam, v = cbk_amd.autotune_compact(p, routine, m, n, nrhs, 64, uplo=uplo, trans=trans)
+        st, fam, v = cbk_amd.autotune_compact(p, routine, m, n, nrhs, 64, nm=TUNE_NM, uplo=uplo,
+                                              trans=trans)
         out[name] = {"status": st, "family": names.get(fam, fam), "V": v,
                      "seconds": time.perf_counter() - t0}
     t0 = time.perf_counter()
-    st, fam, v = cbk_amd.autotune_compact("d", "posv", 32, 32, 4, 0)
+    st, fam, v = cbk_amd.autotune_compact("d", "posv", 32, 32, 4, 0, nm=TUNE_NM)
     out["K4_dposv_n32_pick_V"] = {"status": st, "family": names.get(fam, fam), "V": v,
                                   "seconds": time.perf_counter() - t0}
     log(f"compact tuner: {out}")
@@ -834,10 +843,11 @@ def main():
             "bytes_per_matrix": mixed_bytes,
         },
     }
-    traffic = load_traffic()
-    if traffic:
+    traffic = load_traffic() or {}
+    if "mixed_posv_kernel_dsposv_n32_nrhs4" in traffic:
         result["roofline"]["traffic"] = traffic["mixed_posv_kernel_dsposv_n32_nrhs4"]["bytes_per_launch"]
-        result["plain_fp64"]["traffic_bytes_per_launch"] = traffic["sym_team_kernel_dposv_n32_nrhs4"]["bytes_per_launch"]
+    if "quad_sym_kernel_dposv_n32_nrhs4" in traffic:
+        result["plain_fp64"]["traffic_bytes_per_launch"] = traffic["quad_sym_kernel_dposv_n32_nrhs4"]["bytes_per_launch"]
     if args.calibrate:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -848,16 +858,18 @@ def main():
         result["configs"] = configs_bench(torch)
     if not args.no_uneven:
         # the uneven batch's class thresholds and grid sizes from the tuner
-        log("batch tuner (cbk_autotune) ...")
-        t0 = time.perf_counter()
-        st = cbk_amd.autotune()
-        tuned = cbk_amd.get_tuning()
-        tuned["seconds"] = time.perf_counter() - t0
-        tuned["status"] = st
-        log(f"batch tuning: {tuned}")
+        tuned = None
+        if not args.no_tune:
+            log("batch tuner (cbk_autotune) ...")
+            t0 = time.perf_counter()
+            st = cbk_amd.autotune()
+            tuned = cbk_amd.get_tuning()
+            tuned["seconds"] = time.perf_counter() - t0
+            tuned["status"] = st
+            log(f"batch tuning: {tuned}")
         result["uneven_gels"] = uneven_bench(torch, K=max(2, min(K, 5)), W=1, nprob=args.uneven_nprob)
-        result["uneven_gels"]["tuning"] = tuned
-    if not args.no_configs:
+        result["uneven_gels"]["tuning"] = tuned if tuned else cbk_amd.get_tuning()
+    if not args.no_configs and not args.no_tune:
         result["compact_tuner"] = compact_tuner_bench()
     if not args.no_cpu:
         log("CPU baseline (reference dposv_compact on the host) ...")
