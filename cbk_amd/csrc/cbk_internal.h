@@ -89,7 +89,7 @@ hipError_t launch_reg_chol(bool ldl, bool factor, bool up, int n, int nrhs, Oper
                            Operand<T> B, int V, int nm, int *info);
 
 /* orders served by the potrf/posv/sytrfnp/sysvnp quad kernel (one matrix per
- * 4 lanes, quad_sym.hip): FP64 24 and 32, FP32 32; any V; same contract as
+ * 4 lanes, quad_sym.hip): 24 and 32 in both precisions; any V; same contract as
  * launch_chol_lane. only: if set, the call touches only the matrices with
  * only[mat] < 0 (their A, B and info; the others are left as they are). */
 bool quad_sym_supports(int n, bool fp64);

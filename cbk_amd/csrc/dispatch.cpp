@@ -32,12 +32,14 @@ hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand
         default: break;
         }
     }
-    /* the whole triangle in one lane's registers where it fits (measured
-     * faster than the team kernel at its orders 16 and 24 too) */
-    if (fit && current_policy() != CBK_POLICY_LANE && reg_chol_supports(n, fp64))
-        return launch_reg_chol<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+    /* a matrix per 4 lanes at its orders (at FP32 24 measured faster than the
+     * whole triangle in one lane: ssysvnp nrhs = 24 450 vs 523 us), else the
+     * whole triangle in one lane's registers where it fits (faster than the
+     * team kernel at its orders 16 and 24) */
     if (fit && current_policy() == CBK_POLICY_AUTO && quad_sym_supports(n, fp64))
         return launch_quad_sym<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+    if (fit && current_policy() != CBK_POLICY_LANE && reg_chol_supports(n, fp64))
+        return launch_reg_chol<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     if (fit && current_policy() == CBK_POLICY_AUTO && sym_team_supports(n) && V >= 4)
         return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);

@@ -8,10 +8,15 @@ namespace cbk {
 namespace quadsym {
 extern template hipError_t launch_n<double, 24>(bool, const QuadSymArgs<double> &);
 extern template hipError_t launch_n<double, 32>(bool, const QuadSymArgs<double> &);
+extern template hipError_t launch_n<float, 24>(bool, const QuadSymArgs<float> &);
 extern template hipError_t launch_n<float, 32>(bool, const QuadSymArgs<float> &);
 } /* namespace quadsym */
 
-bool quad_sym_supports(int n, bool fp64) { return fp64 ? (n == 24 || n == 32) : n == 32; }
+bool quad_sym_supports(int n, bool fp64)
+{
+    (void)fp64;
+    return n == 24 || n == 32;
+}
 
 template <class T>
 hipError_t launch_quad_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
@@ -45,6 +50,7 @@ hipError_t launch_quad_sym(bool ldl, bool factor, bool up, int n, int nrhs, Oper
         if (n == 32) return quadsym::launch_n<T, 32>(ldl, args);
     }
     else {
+        if (n == 24) return quadsym::launch_n<T, 24>(ldl, args);
         if (n == 32) return quadsym::launch_n<T, 32>(ldl, args);
     }
     return hipErrorInvalidValue;

@@ -20,9 +20,33 @@ def _gpu():
     return gpuutil
 
 
+class _Register:
+    """route the shapes the quad kernel serves by default (FP32 order 24) to
+    the register kernel through the kernel-family table, for the duration"""
+
+    def __init__(self, p, n, nrhs_list, V_list):
+        self.keys = [(p, r, n, n, nrhs, V) for r in ("posv", "sysvnp") for nrhs in nrhs_list
+                     for V in V_list] if (p, n) == ("s", 24) else []
+
+    def __enter__(self):
+        import cbk_amd
+        for k in self.keys:
+            assert cbk_amd.set_kernel_family(*k, cbk_amd.FAMILY["register"]) == 0
+
+    def __exit__(self, *exc):
+        import cbk_amd
+        for k in self.keys:
+            cbk_amd.set_kernel_family(*k, cbk_amd.FAMILY["default"])
+
+
 @pytest.mark.parametrize("p,n", [(p, n) for p in "ds" for n in (1, 3, 8, 9, 12, 13, 16)] +
                          [("s", 20), ("s", 24)])
 def test_reg_factor_and_solve_bitwise(p, n):
+    with _Register(p, n, (0, 1, 3, 4, 5), (64, 32, 8, 2)):
+        _reg_factor_and_solve_bitwise(p, n)
+
+
+def _reg_factor_and_solve_bitwise(p, n):
     for i, (lay, uplo) in enumerate((("C", "L"), ("C", "U"), ("R", "L"), ("R", "U"))):
         for V, nm in ((64, 130), (32, 71), (8, 13), (2, 5)):
             for rt in ("potrf", "sytrfnp", "posv", "sysvnp", "potrs", "sytrsnp"):
@@ -43,6 +67,11 @@ def test_reg_info(p, n):
     """info = k + 1 for a failed pivot k (Cholesky: a negative diagonal; LDL^T:
     row and column k zeroed up to the diagonal, an exact zero D(k)), every k
     of the order over the batch; failed matrices keep B"""
+    with _Register(p, n, (2,), (64,)):
+        _reg_info(p, n)
+
+
+def _reg_info(p, n):
     V, nm = 64, 70
     for rt in ("posv", "sysvnp"):
         c = cases.make_case(rt, p, {"layout": "C", "uplo": "L"}, {"n": n, "nrhs": 2}, V, nm,
