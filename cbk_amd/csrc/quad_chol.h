@@ -11,7 +11,9 @@
  * form on the scaled column, LDL^T times D(j)), in the same order, so the
  * factor is bit-identical to the reference's; the work is ordered by panels
  * so that A streams in and the factor streams out while it runs. The pivot's
- * own entry becomes scratch; the diagonal is kept apart (dg).
+ * own entry becomes scratch; the diagonal is kept apart (dg). In FP32 the
+ * updates of whole 4x4 tiles can run as MFMA rank-1 updates (mfma_tile,
+ * quad_mfma), which round as fma does, so the factor stays bit-identical.
  *
  * quad_solve: potrs / sytrsnp on NRB right-hand sides held as b[k][q] = B(t +
  * 4k, q). Forward L y = b right-looking per pivot l (y_l from its owner, the
@@ -35,6 +37,34 @@ using team::sqrtT;
 
 template <int N> constexpr int slots() { return N / 4; }
 
+/* Whether the FP32 factor of order N takes its full-tile updates on MFMA
+ * (mfma_tile): measured on 262144 matrices at V = 64, order 24 gains 1.5 to
+ * 3.5 % (spotrf 151.5 -> 146.3 us, sposv nrhs 4 194.2 -> 188.7 us, the mixed
+ * dsposv nrhs 4 781.5 -> 769.7 us), order 32 none (spotrf 251.0 -> 252.3 us,
+ * sposv nrhs 4 305.1 -> 308.9 us); at either order the kernel is bound by
+ * HBM or by its FP64 and solve work, not by the factor's fma's. */
+template <class T, int N> constexpr bool quad_mfma() { return sizeof(T) == 4 && N == 24; }
+
+typedef float mf32x4 __attribute__((ext_vector_type(4)));
+
+/* Tile CB of one slot's row, row[4CB + i] = A(t + 4k, 4CB + i) in lane t,
+ * plus the outer product x y^T, by one v_mfma_f32_4x4x1_16b_f32: its 16
+ * blocks are the wavefront's quads, and register i of lane t of a block
+ * receives C(i, t) + x_i y_t with x_i read from lane i of the quad and y_t
+ * from lane t, so each lane gives its own values and nothing is broadcast.
+ * The instruction rounds x_i y_t + C(i, t) once, as fma does, so it performs
+ * the reference's per-element update bit for bit. */
+template <int CB, int N>
+__device__ __forceinline__ void mfma_tile(float (&row)[N], float x, float y)
+{
+    mf32x4 c = {row[4 * CB], row[4 * CB + 1], row[4 * CB + 2], row[4 * CB + 3]};
+    c = __builtin_amdgcn_mfma_f32_4x4x1f32(x, y, c, 0, 0, 0);
+    row[4 * CB] = c[0];
+    row[4 * CB + 1] = c[1];
+    row[4 * CB + 2] = c[2];
+    row[4 * CB + 3] = c[3];
+}
+
 /* LDS elements of a quad's staging region for the backward sweep: 4 rows of
  * RS = N + 1 (the odd stride spreads a lane's row over the banks), the
  * region padded to 4 mod 32 elements */
@@ -52,11 +82,12 @@ template <int N> constexpr int stage_elems() { return 4 * (N + 1); }
  * next one is factored: load(P) is called before panel P - 1 is computed
  * (the loads of panel P overlap that work), store(P) after panel P is final.
  * N must be a multiple of 8. */
-template <class T, int N, bool LDL, class Load, class Store>
+template <class T, int N, bool LDL, bool MF = false, class Load, class Store>
 __device__ __forceinline__ int quad_factor_panels(T (&a)[N / 4][N], T (&dg)[N / 4], int t,
                                                   Load &&load, Store &&store)
 {
     static_assert(N % 8 == 0, "whole panels");
+    static_assert(!MF || sizeof(T) == 4, "the MFMA tile update is FP32");
     constexpr int K = N / 4;
     int linfo = 0;
     sfor<0, K>([&](auto k) { dg[k] = T(1); });
@@ -71,10 +102,19 @@ __device__ __forceinline__ int quad_factor_panels(T (&a)[N / 4][N], T (&dg)[N / 
             constexpr int j = jc;
             T dj = T(0);
             if constexpr (LDL) dj = qbc<j % 4>(dg[j / 4]);
-            sfor<c0, c1>([&](auto c) {
-                const T w = qbc<c % 4>(LDL ? a[c / 4][j] * dj : a[c / 4][j]);
-                sfor<c / 4, K>([&](auto k) { a[k][c] = fmaT(-w, a[k][j], a[k][c]); });
-            });
+            if constexpr (MF) {
+                /* the panel's two column blocks, tile by tile */
+                sfor<c0 / 4, c1 / 4>([&](auto cb) {
+                    const T x = -(LDL ? a[cb][j] * dj : a[cb][j]);
+                    sfor<cb, K>([&](auto k) { mfma_tile<cb>(a[k], x, a[k][j]); });
+                });
+            }
+            else {
+                sfor<c0, c1>([&](auto c) {
+                    const T w = qbc<c % 4>(LDL ? a[c / 4][j] * dj : a[c / 4][j]);
+                    sfor<c / 4, K>([&](auto k) { a[k][c] = fmaT(-w, a[k][j], a[k][c]); });
+                });
+            }
             __builtin_amdgcn_sched_barrier(0);
         });
         /* the panel, right-looking: panel form, then the column scaled */
@@ -87,10 +127,19 @@ __device__ __forceinline__ int quad_factor_panels(T (&a)[N / 4][N], T (&dg)[N / 
             if constexpr (j + 1 < c1) {
                 T wv[K];
                 sfor<kj, K>([&](auto k) { wv[k] = a[k][j] * inv; });
-                sfor<j + 1, c1>([&](auto c) {
+                /* the columns of j's own block on the VALU; with MF, the
+                 * panel's whole blocks right of it by tiles */
+                constexpr int cv = MF ? cmin(c1, 4 * kj + 4) : c1;
+                sfor<j + 1, cv>([&](auto c) {
                     const T w = qbc<c % 4>(wv[c / 4]);
                     sfor<c / 4, K>([&](auto k) { a[k][c] = fmaT(-w, a[k][j], a[k][c]); });
                 });
+                if constexpr (MF) {
+                    sfor<cv / 4, c1 / 4>([&](auto cb) {
+                        const T x = -wv[cb];
+                        sfor<cb, K>([&](auto k) { mfma_tile<cb>(a[k], x, a[k][j]); });
+                    });
+                }
             }
             T s;
             if constexpr (LDL) {

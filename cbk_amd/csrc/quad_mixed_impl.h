@@ -33,7 +33,7 @@
 #define CBK_QUAD_MIXED_IMPL_H
 
 #include "cbk_internal.h"
-#include "quad_common.h"
+#include "quad_chol.h"
 
 namespace cbk {
 namespace quadmixed {
@@ -163,9 +163,10 @@ __device__ __forceinline__ void sym_abs_rowsum(const double (&a)[N / 4][N], doub
  * D). On return column j holds L(., j) below the diagonal, dinv[k] is 1/L(i,i)
  * or 1/D(i) of the lane's row i = t + 4k; returns whether a pivot failed
  * (Cholesky: not > 0; LDL^T: zero or not finite), alike in the quad's lanes. */
-template <int N, bool LDL>
+template <int N, bool LDL, bool MF = false>
 __device__ __forceinline__ bool factor32(float (&a)[N / 4][N], float (&dinv)[N / 4], int t)
 {
+    static_assert(!MF || N % 4 == 0, "whole tiles");
     constexpr int K = N / 4;
     bool bad = false;
     sfor<0, K>([&](auto k) { dinv[k] = 1.0f; });
@@ -186,10 +187,19 @@ __device__ __forceinline__ bool factor32(float (&a)[N / 4][N], float (&dinv)[N /
         }
         dinv[kj] = (t == tj) ? sc : dinv[kj];
         sfor<kj, K>([&](auto k) { a[k][j] = a[k][j] * sc; });
-        sfor<j + 1, N>([&](auto c) {
+        /* with MF, the column blocks right of j's own by MFMA tiles
+         * (quad::mfma_tile; the same single rounding per element) */
+        constexpr int cv = MF ? 4 * kj + 4 : N;
+        sfor<j + 1, cv>([&](auto c) {
             const float w = LDL ? qbc<c % 4>(a[c / 4][j]) * dmul : qbc<c % 4>(a[c / 4][j]);
             sfor<c / 4, K>([&](auto k) { a[k][c] = __builtin_fmaf(-a[k][j], w, a[k][c]); });
         });
+        if constexpr (MF) {
+            sfor<kj + 1, K>([&](auto cb) {
+                const float x = -(LDL ? a[cb][j] * dmul : a[cb][j]);
+                sfor<cb, K>([&](auto k) { mfma_tile<cb>(a[k], x, a[k][j]); });
+            });
+        }
         __builtin_amdgcn_sched_barrier(0);
     });
     return bad;
@@ -292,7 +302,7 @@ quad_mixed_kernel(QuadMixedArgs p)
                                          (unsigned)c * a_col);
             });
         });
-        if (factor32<N, LDL>(a32, dinv, t)) {
+        if (factor32<N, LDL, quad_mfma<float, N>()>(a32, dinv, t)) {
             iter = -3;
             failed = true;
         }

@@ -145,7 +145,7 @@ quad_sym_kernel(QuadSymArgs<T> p)
                 bstore(dg[k], ra, doff, 0);
             });
         };
-        linfo = quad_factor_panels<T, N, LDL>(a, dg, t, load, store);
+        linfo = quad_factor_panels<T, N, LDL, quad_mfma<T, N>()>(a, dg, t, load, store);
         if (p.info && pl.live && pl.mat < p.nm && t == 0) p.info[pl.mat] = linfo;
         if constexpr (!prefetch_b<T, N>()) {
             if (p.nrhs > 0) load_b(b, qfirst);
