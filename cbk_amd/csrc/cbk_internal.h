@@ -250,6 +250,11 @@ hipError_t launch_gels_batch_lane(const BatchArgs<T> &args, hipStream_t s);
 /* gels on compact operands through the wavefront kernel: the tall view p x q
  * (transposed: A^T), p <= 64, q + nrhs <= 64, nrhs >= 1, any V */
 bool qr_wave_supports(int p, int q, int nrhs);
+/* FP64 gels on a 4 x 8 lane grid per problem, two problems per wavefront
+ * (qr_grid.hip): p <= 64, q <= 32, nrhs <= 8 */
+bool qr_grid_supports(int p, int q, int nrhs);
+hipError_t launch_qr_grid(int p, int q, int nrhs, bool transposed, Operand<double> A, double *tau,
+                          Operand<double> B, int V, int nm, int *info);
 template <class T>
 hipError_t launch_qr_wave(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
                           Operand<T> B, int V, int nm, int *info);

@@ -139,6 +139,10 @@ hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operan
         return launch_qr_team<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
     if (current_policy() == CBK_POLICY_AUTO && overdet && V >= 4 && qr_cols_supports(p, q, nrhs))
         return launch_qr_cols<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
+    if constexpr (sizeof(T) == 8) {
+        if (current_policy() != CBK_POLICY_LANE && overdet && qr_grid_supports(p, q, nrhs))
+            return launch_qr_grid(p, q, nrhs, !tall, A, tau, B, V, nm, info);
+    }
     if (current_policy() != CBK_POLICY_LANE && overdet && qr_wave_supports(p, q, nrhs))
         return launch_qr_wave<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
     return launch_gels_lane<T>(tran, m, n, nrhs, A, B, tau, V, nm, info);
