@@ -137,50 +137,100 @@ qr_grid_kernel(GridArgs p)
     });
 
     double tauv[CSA] = {0.0, 0.0, 0.0, 0.0}; /* tau of reflector c + 8b */
-    sfor<0, KMAX>([&](auto kc) {
+
+    /* reflector kk from column kk (slot kk / 8 of column lane kk % 8),
+     * which has received every earlier reflector: its squares below row kk
+     * (two partial sums a lane, then the quad), larfg, the body scaled in
+     * place, R(kk, kk) and tau; returns tau (valid in column lane kk % 8) */
+    auto gen = [&](auto kc) {
         constexpr int kk = kc;
         constexpr int ak = kk / 4, rk = kk % 4, bk = kk / 8, ck = kk % 8;
+        double s0 = (r > rk) ? f[ak][bk] * f[ak][bk] : 0.0, s1 = 0.0;
+        sfor<ak + 1, RS>([&](auto a) {
+            if constexpr ((a - ak) % 2 == 1)
+                s1 = fmaT(f[a][bk], f[a][bk], s1);
+            else
+                s0 = fmaT(f[a][bk], f[a][bk], s0);
+        });
+        const double s = quad_sum(s0 + s1);
+        const batch::Larfg<double> hh = batch::larfg(qbc<rk>(f[ak][bk]), s);
+        const bool own = c == ck;
+        const double sc = own ? hh.inv : 1.0; /* x 1.0 is exact */
+        sfor<ak + 1, RS>([&](auto a) { f[a][bk] = f[a][bk] * sc; });
+        f[ak][bk] = own ? ((r > rk) ? f[ak][bk] * hh.inv : ((r == rk) ? hh.rdiag : f[ak][bk]))
+                        : f[ak][bk];
+        tauv[bk] = own ? hh.tau : tauv[bk];
+        return hh.tau;
+    };
+    /* v of reflector kk on this lane's rows (v_kk = 1, rows above kk 0) and
+     * tau, from column lane kk % 8 */
+    double vv[RS];
+    auto bcast = [&](auto kc, double tl) {
+        constexpr int kk = kc;
+        constexpr int ak = kk / 4, rk = kk % 4, bk = kk / 8, ck = kk % 8;
+        sfor<ak + 1, RS>([&](auto a) { vv[a] = colbc<ck>(f[a][bk]); });
+        const double v0 = colbc<ck>(f[ak][bk]);
+        vv[ak] = (r > rk) ? v0 : ((r == rk) ? 1.0 : 0.0);
+        return colbc<ck>(tl);
+    };
+    /* reflector kk on A's column slot b (b >= kk / 8): w = tau v^T A(:, j)
+     * on this lane's rows, then over the quad (columns j <= kk get w = 0, the
+     * wavefront kernel's masking); then A(i, j) -= v_i w_j on rows i >= kk,
+     * the rows above kk in slot kk / 4 keeping their values. B's column is
+     * slot CSA. */
+    auto dot = [&](auto kc, auto bc, double tau) {
+        constexpr int kk = kc, b = bc;
+        constexpr int ak = kk / 4, bk = kk / 8, ck = kk % 8;
+        double d;
+        if constexpr (b < CSA) {
+            d = vv[ak] * f[ak][b];
+            sfor<ak + 1, RS>([&](auto a) { d = fmaT(vv[a], f[a][b], d); });
+        }
+        else {
+            d = vv[ak] * gb[ak];
+            sfor<ak + 1, RS>([&](auto a) { d = fmaT(vv[a], gb[a], d); });
+        }
+        d = quad_sum(d);
+        if constexpr (b == CSA) return d * tau;
+        return (b > bk || c > ck) ? d * tau : 0.0;
+    };
+    auto upd = [&](auto kc, auto bc, double w) {
+        constexpr int kk = kc, b = bc;
+        constexpr int ak = kk / 4, rk = kk % 4;
+        if constexpr (b < CSA) {
+            f[ak][b] = (r >= rk) ? fmaT(-vv[ak], w, f[ak][b]) : f[ak][b];
+            sfor<ak + 1, RS>([&](auto a) { f[a][b] = fmaT(-vv[a], w, f[a][b]); });
+        }
+        else {
+            gb[ak] = (r >= rk) ? fmaT(-vv[ak], w, gb[ak]) : gb[ak];
+            sfor<ak + 1, RS>([&](auto a) { gb[a] = fmaT(-vv[a], w, gb[a]); });
+        }
+    };
+    /* Look-ahead: step kk first finishes the slot that holds column kk + 1,
+     * then generates reflector kk + 1 from it in the same basic block as the
+     * updates of the other slots and of B, so that the scheduler overlaps its
+     * sums, square root and divisions with them; its v is broadcast once
+     * reflector kk's v is dead. Past
+     * column n - 1 the generated reflector comes from a zero column (tau = 0,
+     * nothing scaled) and is never stored. */
+    double tau = 0.0;
+    if (n > 0) tau = bcast(std::integral_constant<int, 0>{}, gen(std::integral_constant<int, 0>{}));
+    sfor<0, KMAX>([&](auto kc) {
+        constexpr int kk = kc;
+        constexpr int bk = kk / 8, b1 = (kk + 1) / 8;
         if (kk < n) {
-            /* the pivot column's squares below row kk, row order per lane */
-            double s = (r > rk) ? f[ak][bk] * f[ak][bk] : 0.0;
-            sfor<ak + 1, RS>([&](auto a) { s = fmaT(f[a][bk], f[a][bk], s); });
-            s = quad_sum(s);
-            const batch::Larfg<double> hh = batch::larfg(qbc<rk>(f[ak][bk]), s);
-            const bool own = c == ck;
-            const double sc = own ? hh.inv : 1.0; /* x 1.0 is exact */
-            sfor<ak + 1, RS>([&](auto a) { f[a][bk] = f[a][bk] * sc; });
-            f[ak][bk] = own ? ((r > rk) ? f[ak][bk] * hh.inv : ((r == rk) ? hh.rdiag : f[ak][bk]))
-                            : f[ak][bk];
-            tauv[bk] = own ? hh.tau : tauv[bk];
-            /* v on this lane's rows (v_kk = 1, rows above kk 0) and tau */
-            double vv[RS];
-            sfor<ak + 1, RS>([&](auto a) { vv[a] = colbc<ck>(f[a][bk]); });
-            const double v0 = colbc<ck>(f[ak][bk]);
-            vv[ak] = (r > rk) ? v0 : ((r == rk) ? 1.0 : 0.0);
-            const double tau = colbc<ck>(hh.tau);
-            /* w = v^T [A | B] on the columns right of kk, this lane's rows,
-             * then over the quad */
-            double w[CSA], wb;
-            sfor<bk, CSA>([&](auto b) {
-                double d = vv[ak] * f[ak][b];
-                sfor<ak + 1, RS>([&](auto a) { d = fmaT(vv[a], f[a][b], d); });
-                d = quad_sum(d);
-                w[b] = (b > bk || c > ck) ? d * tau : 0.0;
-            });
-            {
-                double d = vv[ak] * gb[ak];
-                sfor<ak + 1, RS>([&](auto a) { d = fmaT(vv[a], gb[a], d); });
-                wb = quad_sum(d) * tau;
+            if constexpr (kk + 1 < KMAX) {
+                constexpr auto B1 = std::integral_constant<int, b1>{};
+                upd(kc, B1, dot(kc, B1, tau));
+                const double tn = gen(std::integral_constant<int, kk + 1>{});
+                sfor<bk, CSA + 1>([&](auto b) {
+                    if constexpr (b != b1) upd(kc, b, dot(kc, b, tau));
+                });
+                tau = bcast(std::integral_constant<int, kk + 1>{}, tn);
             }
-            /* A(i, j) -= v_i w_j on rows i >= kk; columns j <= kk have w = 0
-             * (the wavefront kernel's masking), and the rows above kk in
-             * slot ak keep their values */
-            sfor<bk, CSA>([&](auto b) {
-                f[ak][b] = (r >= rk) ? fmaT(-vv[ak], w[b], f[ak][b]) : f[ak][b];
-                sfor<ak + 1, RS>([&](auto a) { f[a][b] = fmaT(-vv[a], w[b], f[a][b]); });
-            });
-            gb[ak] = (r >= rk) ? fmaT(-vv[ak], wb, gb[ak]) : gb[ak];
-            sfor<ak + 1, RS>([&](auto a) { gb[a] = fmaT(-vv[a], wb, gb[a]); });
+            else {
+                sfor<bk, CSA + 1>([&](auto b) { upd(kc, b, dot(kc, b, tau)); });
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
     });
