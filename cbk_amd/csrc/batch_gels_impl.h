@@ -649,6 +649,8 @@ template <class T> hipError_t launch_class(int cls, const BatchArgs<T> &args, un
     case 5: hipLaunchKernelGGL((gels_wave2_kernel<T>), grid, block, 0, s, args); break;
     case 7: hipLaunchKernelGGL((gels_wave_pair_kernel<T, 32>), grid, block, 0, s, args); break;
     case 8: hipLaunchKernelGGL((gels_wave_pair_kernel<T, 64>), grid, block, 0, s, args); break;
+    case 9:
+    case 10: return launch_qr_grid_batch<T>(cls, args, blocks, s);
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -720,6 +722,8 @@ template <class T> hipError_t gels_batch_occupancy(int cls, int *blocks)
     case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave2_kernel<T>, 256, 0);
     case 7: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_pair_kernel<T, 32>, 256, 0);
     case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_pair_kernel<T, 64>, 256, 0);
+    case 9:
+    case 10: return qr_grid_batch_occupancy<T>(cls, blocks);
     default: *blocks = 0; return hipErrorInvalidValue;
     }
 }

@@ -92,13 +92,15 @@ double work_of(const cbk_gels_problem &p)
 }
 
 /* the kernel a problem of class c runs on: the wavefront classes' problems
- * with n + nrhs <= 32 go two per wavefront (kernels 7 and 8) */
+ * with n <= 32 and nrhs <= 8 on the lane-grid kernel (9, 10), their other
+ * problems with n + nrhs <= 32 two per wavefront (7, 8) */
 int kernel_of(int c, const cbk_gels_problem &p)
 {
+    if ((c == 3 || c == 4) && p.n <= 32 && p.nrhs <= 8) return c + 6;
     return (c == 3 || c == 4) && p.n + p.nrhs <= 32 ? c + 4 : c;
 }
 /* the public class of kernel k */
-int class_of_kernel(int k) { return k >= kBatchClasses ? k - 4 : k; }
+int class_of_kernel(int k) { return k >= 9 ? k - 6 : k >= kBatchClasses ? k - 4 : k; }
 
 int problems_per_block(int k) { return k <= 2 ? 16 : k <= 5 ? 4 : k == kBatchGeneral ? 256 : 8; }
 
@@ -839,8 +841,8 @@ int cbk_autotune(void)
             const int nrhs = 2, n = w - nrhs, m = rows[b];
             float tn = 0.f, tw = 0.f;
             e = time_class<double>(b, m, n, nrhs, 16384, dev, &tn);
-            /* the wavefront kernel such a problem would take: two per wavefront */
-            if (e == hipSuccess) e = time_class<double>(m <= 32 ? 7 : 8, m, n, nrhs, 16384, dev, &tw);
+            /* the kernel such a problem would take otherwise: the lane grid */
+            if (e == hipSuccess) e = time_class<double>(m <= 32 ? 9 : 10, m, n, nrhs, 16384, dev, &tw);
             if (e != hipSuccess) return cbk::launch_status(e);
             if (tn > tw) break;
             maxw = w;
@@ -849,8 +851,8 @@ int cbk_autotune(void)
     }
     /* grid sizes: class, the kernel behind it at the shape, m, n, nrhs, count */
     const int shapes[6][6] = {{0, 0, 16, 6, 2, 16384},  {1, 1, 32, 6, 2, 16384},
-                              {2, 2, 64, 6, 2, 16384},  {3, 7, 32, 20, 4, 16384},
-                              {4, 8, 64, 24, 4, 16384}, {5, 5, 64, 60, 6, 4096}};
+                              {2, 2, 64, 6, 2, 16384},  {3, 9, 32, 20, 4, 16384},
+                              {4, 10, 64, 24, 4, 16384}, {5, 5, 64, 60, 6, 4096}};
     for (const auto &sh : shapes) {
         int occ = 0;
         e = cbk::gels_batch_occupancy<double>(sh[1], &occ);

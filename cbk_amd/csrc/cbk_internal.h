@@ -240,13 +240,19 @@ template <class T> __device__ __forceinline__ unsigned batch_count(const BatchAr
  * lane kernel */
 constexpr int kBatchClasses = 7;
 constexpr int kBatchGeneral = 6;
-/* kernels behind the classes: 0-6 as the classes, and 7 / 8 the wavefront
- * classes 3 / 4's problems with n + nrhs <= 32, two per wavefront */
-constexpr int kBatchKernels = 9;
+/* kernels behind the classes: 0-6 as the classes; 9 / 10 the wavefront
+ * classes 3 / 4's problems with n <= 32 and nrhs <= 8 on the lane-grid kernel
+ * (qr_grid.hip), two per wavefront; 7 / 8 their other problems with
+ * n + nrhs <= 32, two per wavefront */
+constexpr int kBatchKernels = 11;
 template <class T>
 hipError_t launch_gels_batch(int cls, const BatchArgs<T> &args, unsigned blocks, hipStream_t s);
 template <class T>
 hipError_t launch_gels_batch_lane(const BatchArgs<T> &args, hipStream_t s);
+/* batch kernels 9 and 10 (qr_grid.hip) */
+template <class T>
+hipError_t launch_qr_grid_batch(int kernel, const BatchArgs<T> &args, unsigned blocks, hipStream_t s);
+template <class T> hipError_t qr_grid_batch_occupancy(int kernel, int *blocks);
 /* gels on compact operands through the wavefront kernel: the tall view p x q
  * (transposed: A^T), p <= 64, q + nrhs <= 64, nrhs >= 1, any V */
 bool qr_wave_supports(int p, int q, int nrhs);

@@ -2,8 +2,9 @@
 oracle's batch gels on the same pools.
 
 Every kernel class is exercised: the team kernel (n + nrhs <= 16; rows <= 16,
-32, 64), the wavefront kernel (n + nrhs <= 64; rows <= 32, 64; two problems per
-wavefront when n + nrhs <= 32, one otherwise), the
+32, 64), the wavefront classes (n + nrhs <= 64; rows <= 32, 64: the lane-grid
+kernel for n <= 32 and nrhs <= 8, two problems per wavefront; otherwise two
+problems per wavefront when n + nrhs <= 32, one otherwise), the
 two-array wavefront kernel (n + nrhs > 64) and the general lane kernel
 (transposed, underdetermined, more than 64 rows, empty problems). The general kernel performs the oracle's
 operations and is compared bit for bit; the register kernels sum in another
