@@ -291,7 +291,8 @@ def autotune():
     return _bind_batch().cbk_autotune()
 
 
-FAMILY = {"default": 0, "register": 1, "quad": 2, "team": 3, "columns": 4, "wave": 5, "lane": 6}
+FAMILY = {"default": 0, "register": 1, "quad": 2, "team": 3, "columns": 4, "wave": 5, "lane": 6,
+          "grid": 7}
 
 
 def _bind_tune():

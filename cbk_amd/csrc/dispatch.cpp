@@ -125,6 +125,12 @@ hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operan
             if (qr_wave_supports(p, q, nrhs))
                 return launch_qr_wave<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
             break;
+        case CBK_FAMILY_GRID:
+            if constexpr (sizeof(T) == 8) {
+                if (qr_grid_supports(p, q, nrhs))
+                    return launch_qr_grid(p, q, nrhs, !tall, A, tau, B, V, nm, info);
+            }
+            break;
         case CBK_FAMILY_LANE: return launch_gels_lane<T>(tran, m, n, nrhs, A, B, tau, V, nm, info);
         default: break;
         }
@@ -167,6 +173,12 @@ template <class T> hipError_t dispatch_geqrf(int m, int n, Operand<T> A, T *tau,
             if (V >= 4 && qr_cols_supports(m, n, 0))
                 return launch_qr_cols<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
             break;
+        case CBK_FAMILY_GRID:
+            if constexpr (sizeof(T) == 8) {
+                if (qr_grid_supports(m, n, 0))
+                    return launch_qr_grid(m, n, 0, false, A, tau, none, V, nm, nullptr);
+            }
+            break;
         case CBK_FAMILY_LANE: return launch_geqrf_lane<T>(m, n, A, tau, V, nm);
         default: break;
         }
@@ -178,6 +190,10 @@ template <class T> hipError_t dispatch_geqrf(int m, int n, Operand<T> A, T *tau,
         return launch_qr_team<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
     if (current_policy() == CBK_POLICY_AUTO && tall && V >= 4 && qr_cols_supports(m, n, 0))
         return launch_qr_cols<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
+    if constexpr (sizeof(T) == 8) {
+        if (current_policy() != CBK_POLICY_LANE && tall && qr_grid_supports(m, n, 0))
+            return launch_qr_grid(m, n, 0, false, A, tau, none, V, nm, nullptr);
+    }
     return launch_geqrf_lane<T>(m, n, A, tau, V, nm);
 }
 

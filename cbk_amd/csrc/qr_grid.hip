@@ -321,7 +321,7 @@ qr_grid_kernel(GridArgs p)
     /* LAPACK dgels stops at a singular R (B keeps Q^T B); without info the
      * substitution runs as the reference's gels_compact. bad is uniform in
      * the half. */
-    if (!(p.info && bad)) back_substitute<double, RS>(f, gb, r, n);
+    if (nrhs > 0 && !(p.info && bad)) back_substitute<double, RS>(f, gb, r, n);
     sfor<0, RS>([&](auto a) { team::bstore(gb[a], rb, b_at(boff2, r2, c2, a), a * 4 * brs); });
 }
 
@@ -378,7 +378,11 @@ qr_grid_batch_kernel(BatchArgs<T> p)
 
 } /* namespace qrgrid */
 
-bool qr_grid_supports(int p, int q, int nrhs) { return p <= 64 && q <= 32 && q <= p && nrhs <= 8; }
+/* gels with nrhs <= 8, or geqrf (nrhs = 0), of a tall p x q view */
+bool qr_grid_supports(int p, int q, int nrhs)
+{
+    return p <= 64 && q >= 1 && q <= 32 && q <= p && nrhs >= 0 && nrhs <= 8;
+}
 
 hipError_t launch_qr_grid(int p, int q, int nrhs, bool transposed, Operand<double> A, double *tau,
                           Operand<double> B, int V, int nm, int *info)

@@ -120,6 +120,7 @@ std::vector<int> candidates(const Shape &s, int V)
         if (V >= 4 && qr_team_supports(s.p, s.q, r, s.fp64)) f.push_back(CBK_FAMILY_TEAM);
         if (V >= 4 && qr_cols_supports(s.p, s.q, r)) f.push_back(CBK_FAMILY_COLUMNS);
         if (s.cls == kTuneGels && qr_wave_supports(s.p, s.q, r)) f.push_back(CBK_FAMILY_WAVE);
+        if (s.fp64 && qr_grid_supports(s.p, s.q, r)) f.push_back(CBK_FAMILY_GRID);
         break;
     }
     }
@@ -337,7 +338,7 @@ int cbk_set_kernel_family(char prec, const char *routine, char uplo, char trans,
     const int code = shape_of(prec, routine, uplo, trans, m, n, nrhs, s);
     if (code != 0) return code;
     if (!vlen_ok(V)) return -8;
-    if (family < CBK_FAMILY_DEFAULT || family > CBK_FAMILY_LANE) return -10;
+    if (family < CBK_FAMILY_DEFAULT || family > CBK_FAMILY_GRID) return -10;
     put_family(s.cls, s.fp64, s.p, s.q, s.nrhs, V, family);
     return 0;
 }

@@ -28,7 +28,9 @@ SHAPES = [
     ("s", "trsm", 16, 8, 0, "U", "N", "trsm",
      {"layout": "C", "side": "L", "uplo": "U", "trans": "N", "diag": "N"}, {"m": 16, "n": 8}),
     ("d", "gels", 16, 8, 2, "L", "N", "gels", {"layout": "C", "trans": "N"}, {"m": 16, "n": 8, "nrhs": 2}),
+    ("d", "gels", 64, 32, 2, "L", "N", "gels", {"layout": "C", "trans": "N"}, {"m": 64, "n": 32, "nrhs": 2}),
     ("d", "geqrf", 32, 10, 0, "L", "N", "geqrf", {"layout": "C"}, {"m": 32, "n": 10}),
+    ("d", "geqrf", 64, 32, 0, "L", "N", "geqrf", {"layout": "C"}, {"m": 64, "n": 32}),
 ]
 
 
@@ -45,7 +47,7 @@ def test_tuner_installs_a_family_and_every_family_is_right(shape):
     try:
         st, fam, v = cbk_amd.autotune_compact(p, routine, m, n, nrhs, V, nm=16384, uplo=uplo,
                                               trans=trans)
-        assert st == 0 and v == V and F["register"] <= fam <= F["lane"], (st, fam, v)
+        assert st == 0 and v == V and F["register"] <= fam <= F["grid"], (st, fam, v)
         assert cbk_amd.get_kernel_family(p, routine, m, n, nrhs, V, uplo=uplo, trans=trans) == fam
         print("tuned family", fam, flush=True)
         # every family forced: the oracle's results
@@ -53,7 +55,8 @@ def test_tuner_installs_a_family_and_every_family_is_right(shape):
         case = cases.make_case(crout, p, opts, dims, V, nm, seed=77)
         so, outs_o = cases.run(cbkt.oracle().call, case)
         assert so == 0
-        for f in (F["register"], F["quad"], F["team"], F["columns"], F["wave"], F["lane"]):
+        for f in (F["register"], F["quad"], F["team"], F["columns"], F["wave"], F["lane"],
+                  F["grid"]):
             assert cbk_amd.set_kernel_family(p, routine, m, n, nrhs, V, f, uplo=uplo,
                                              trans=trans) == 0
             sg, outs_g = cases.run(gpuutil.gpu_call, case)

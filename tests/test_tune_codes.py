@@ -34,7 +34,7 @@ def test_family_argument_codes():
     assert cbk_amd.set_kernel_family("d", "gels", 8, 9, 1, 64, 1) == -6
     assert cbk_amd.set_kernel_family("d", "gels", 16, 8, 0, 64, 1) == -7
     assert cbk_amd.set_kernel_family("d", "posv", 8, 8, 1, 12, 1) == -8
-    assert cbk_amd.set_kernel_family("d", "posv", 8, 8, 1, 64, 7) == -10
+    assert cbk_amd.set_kernel_family("d", "posv", 8, 8, 1, 64, 8) == -10
     assert cbk_amd.get_kernel_family("d", "geqrf", 4, 8, 0, 64) == -6
     assert cbk_amd.autotune_compact("d", "posv", 8, 8, 1, 12)[0] == -8
     assert cbk_amd.autotune_compact("d", "posv", 8, 8, 1, 64, nm=-1)[0] == -9
