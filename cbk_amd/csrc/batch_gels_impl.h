@@ -651,6 +651,7 @@ template <class T> hipError_t launch_class(int cls, const BatchArgs<T> &args, un
     case 8: hipLaunchKernelGGL((gels_wave_pair_kernel<T, 64>), grid, block, 0, s, args); break;
     case 9:
     case 10: return launch_qr_grid_batch<T>(cls, args, blocks, s);
+    case 11: return launch_qr_grid64_batch<T>(args, blocks, s);
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -724,6 +725,7 @@ template <class T> hipError_t gels_batch_occupancy(int cls, int *blocks)
     case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, gels_wave_pair_kernel<T, 64>, 256, 0);
     case 9:
     case 10: return qr_grid_batch_occupancy<T>(cls, blocks);
+    case 11: return qr_grid64_batch_occupancy<T>(blocks);
     default: *blocks = 0; return hipErrorInvalidValue;
     }
 }

@@ -92,17 +92,26 @@ double work_of(const cbk_gels_problem &p)
 }
 
 /* the kernel a problem of class c runs on: the wavefront classes' problems
- * with n <= 32 and nrhs <= 8 on the lane-grid kernel (9, 10), their other
- * problems with n + nrhs <= 32 two per wavefront (7, 8) */
+ * with n <= 32 and nrhs <= 8 on the lane-grid kernel (9, 10), two per
+ * wavefront; the wider ones of classes 4 and 5 with nrhs <= 8 on the 8 x 8
+ * lane grid (11), one per wavefront; the other problems of classes 3 and 4
+ * with n + nrhs <= 32 two per wavefront (7, 8) */
 int kernel_of(int c, const cbk_gels_problem &p)
 {
     if ((c == 3 || c == 4) && p.n <= 32 && p.nrhs <= 8) return c + 6;
+    if ((c == 4 || c == 5) && p.nrhs <= 8) return 11;
     return (c == 3 || c == 4) && p.n + p.nrhs <= 32 ? c + 4 : c;
 }
-/* the public class of kernel k */
-int class_of_kernel(int k) { return k >= 9 ? k - 6 : k >= kBatchClasses ? k - 4 : k; }
+/* the public class of kernel k (kernel 11 is tuned as class 5) */
+int class_of_kernel(int k)
+{
+    return k == 11 ? 5 : k >= 9 ? k - 6 : k >= kBatchClasses ? k - 4 : k;
+}
 
-int problems_per_block(int k) { return k <= 2 ? 16 : k <= 5 ? 4 : k == kBatchGeneral ? 256 : 8; }
+int problems_per_block(int k)
+{
+    return k <= 2 ? 16 : k <= 5 || k == 11 ? 4 : k == kBatchGeneral ? 256 : 8;
+}
 
 /* a problem the mixed path refines: trans 'N', 1 <= n <= m <= 64, n <= 32,
  * 1 <= nrhs <= 8 */
@@ -852,7 +861,7 @@ int cbk_autotune(void)
     /* grid sizes: class, the kernel behind it at the shape, m, n, nrhs, count */
     const int shapes[6][6] = {{0, 0, 16, 6, 2, 16384},  {1, 1, 32, 6, 2, 16384},
                               {2, 2, 64, 6, 2, 16384},  {3, 9, 32, 20, 4, 16384},
-                              {4, 10, 64, 24, 4, 16384}, {5, 5, 64, 60, 6, 4096}};
+                              {4, 10, 64, 24, 4, 16384}, {5, 11, 64, 60, 6, 4096}};
     for (const auto &sh : shapes) {
         int occ = 0;
         e = cbk::gels_batch_occupancy<double>(sh[1], &occ);

@@ -242,9 +242,11 @@ constexpr int kBatchClasses = 7;
 constexpr int kBatchGeneral = 6;
 /* kernels behind the classes: 0-6 as the classes; 9 / 10 the wavefront
  * classes 3 / 4's problems with n <= 32 and nrhs <= 8 on the lane-grid kernel
- * (qr_grid.hip), two per wavefront; 7 / 8 their other problems with
- * n + nrhs <= 32, two per wavefront */
-constexpr int kBatchKernels = 11;
+ * (qr_grid.hip), two per wavefront; 11 the wider problems of classes 4 and 5
+ * with nrhs <= 8 on the 8 x 8 lane grid (qr_grid64.hip), one per wavefront;
+ * 7 / 8 the classes 3 / 4's other problems with n + nrhs <= 32, two per
+ * wavefront */
+constexpr int kBatchKernels = 12;
 template <class T>
 hipError_t launch_gels_batch(int cls, const BatchArgs<T> &args, unsigned blocks, hipStream_t s);
 template <class T>
@@ -253,6 +255,10 @@ hipError_t launch_gels_batch_lane(const BatchArgs<T> &args, hipStream_t s);
 template <class T>
 hipError_t launch_qr_grid_batch(int kernel, const BatchArgs<T> &args, unsigned blocks, hipStream_t s);
 template <class T> hipError_t qr_grid_batch_occupancy(int kernel, int *blocks);
+/* batch kernel 11 (qr_grid64.hip) */
+template <class T>
+hipError_t launch_qr_grid64_batch(const BatchArgs<T> &args, unsigned blocks, hipStream_t s);
+template <class T> hipError_t qr_grid64_batch_occupancy(int *blocks);
 /* gels on compact operands through the wavefront kernel: the tall view p x q
  * (transposed: A^T), p <= 64, q + nrhs <= 64, nrhs >= 1, any V */
 bool qr_wave_supports(int p, int q, int nrhs);

@@ -1,0 +1,292 @@
+/* qr_grid64.hip -- gels for the uneven batches' wide problems: m <= 64,
+ * 32 < n <= 64 or n + nrhs > 64, nrhs <= 8 (the reference's geqrf + ormqr +
+ * trsm path of gels, src/cbk_gels_compact.hpp:124-160,
+ * src/cbk_geqrf_compact.hpp:74-131, src/cbk_ormqr_compact.hpp:68-105), one
+ * problem per wavefront on an 8 x 8 grid of lanes.
+ *
+ * Lane (r, c) = lane bits [2:0] and [5:3] holds rows i = r + 8a (a < 8) of the
+ * columns j = c + 8b (b < 8) of A, f[a][b], and rows r + 8a of B's column c
+ * (c < nrhs), gb[a]: 72 values a lane. Per reflector kk (rows and slots
+ * compile-time; the pivot row and column both sit in slot kk / 8 of lane
+ * kk % 8):
+ *   gen    column lane kk % 8 sums the squares below row kk (row order per
+ *          lane, then over its 8 row lanes: quad_perm and row_half_mirror DPP
+ *          adds); x0 = A(kk, kk) and the sum reach every lane by v_readlane,
+ *          so larfg runs on wave-uniform values (batch_gels_impl.h's larfg);
+ *   bcast  the scaled body goes from column lane kk % 8 to the others by
+ *          ds_bpermute (the column lanes span both halves of the wavefront);
+ *   apply  each lane's partial w_j over its rows, the 8-row-lane DPP sum,
+ *          then A(i, j) -= v_i tau w_j on its own rows; B's column likewise.
+ * Step kk first finishes the slot that holds column kk + 1 and generates
+ * reflector kk + 1 from it while the other slots are updated (look-ahead).
+ * R X = (Q^T B)(0:n) then runs column-oriented from row n-1 up.
+ *
+ * Sums run per lane in row order and then over the row lanes, not in the
+ * reference's single row order, so results agree with it to rounding (the
+ * tests bound it, as for the wavefront kernels). Where the one-problem
+ * wavefront kernels run a 64-row fma chain per column lane and reflector, a
+ * lane here runs at most 8 rows, and the wavefront's 64 lanes all hold data.
+ */
+#include "batch_gels_impl.h"
+#include "cbk_internal.h"
+#include "quad_common.h"
+
+namespace cbk {
+namespace qrgrid64 {
+
+using team::dpp;
+using team::fmaT;
+using team::sfor;
+using team::sfor_down;
+using quad::qx;
+
+constexpr int RS = 8; /* row slots: rows r + 8a */
+constexpr int CS = 8; /* column slots: columns c + 8b */
+constexpr int KMAX = 64;
+
+/* the sum over the 8 row lanes of this lane's column lane, in each of them */
+template <class T> __device__ __forceinline__ T row_sum(T x)
+{
+    x = x + qx<1>(x);
+    x = x + qx<2>(x);
+    return x + dpp<0x141>(x); /* row_half_mirror: the other quad of the 8 */
+}
+
+/* value of lane (r, C) (lane r + 8C), for every lane (r, c) */
+template <int C> __device__ __forceinline__ double colbc(double x, int raddr)
+{
+    int2 w = __builtin_bit_cast(int2, x);
+    w.x = __builtin_amdgcn_ds_bpermute(raddr + 32 * C, w.x);
+    w.y = __builtin_amdgcn_ds_bpermute(raddr + 32 * C, w.y);
+    return __builtin_bit_cast(double, w);
+}
+template <int C> __device__ __forceinline__ float colbc(float x, int raddr)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(raddr + 32 * C, __builtin_bit_cast(int, x)));
+}
+
+/* value of lane (R, c) of this lane's column lane (ds_swizzle within 32
+ * lanes: and 0x18, or R) */
+template <int R> __device__ __forceinline__ double rowbc(double x)
+{
+    constexpr int pat = 0x18 | (R << 5);
+    int2 w = __builtin_bit_cast(int2, x);
+    w.x = __builtin_amdgcn_ds_swizzle(w.x, pat);
+    w.y = __builtin_amdgcn_ds_swizzle(w.y, pat);
+    return __builtin_bit_cast(double, w);
+}
+template <int R> __device__ __forceinline__ float rowbc(float x)
+{
+    constexpr int pat = 0x18 | (R << 5);
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, x), pat));
+}
+
+/* x of lane L, in every lane (wave-uniform) */
+template <int L> __device__ __forceinline__ double lane_value(double x)
+{
+    int2 w = __builtin_bit_cast(int2, x);
+    w.x = __builtin_amdgcn_readlane(w.x, L);
+    w.y = __builtin_amdgcn_readlane(w.y, L);
+    return __builtin_bit_cast(double, w);
+}
+template <int L> __device__ __forceinline__ float lane_value(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), L));
+}
+
+template <class T> __device__ __forceinline__ T opaque(T x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+/* QR of the problem's n columns (rows past the problem zero), Q^T B in gb as
+ * it goes */
+template <class T>
+__device__ __forceinline__ void factor(T (&f)[RS][CS], T (&gb)[RS], int r, int c, int n, int raddr)
+{
+    auto gen = [&](auto kc) {
+        constexpr int kk = kc;
+        constexpr int ak = kk / 8, rk = kk % 8; /* row slot / lane; column slot / lane likewise */
+        T s0 = (r > rk) ? f[ak][ak] * f[ak][ak] : T(0), s1 = T(0);
+        sfor<ak + 1, RS>([&](auto a) {
+            if constexpr ((a - ak) % 2 == 1)
+                s1 = fmaT(f[a][ak], f[a][ak], s1);
+            else
+                s0 = fmaT(f[a][ak], f[a][ak], s0);
+        });
+        const T s = row_sum(s0 + s1);
+        /* lane (rk, rk) holds A(kk, kk); lane (0, rk) the column's sum */
+        const batch::Larfg<T> hh =
+            batch::larfg(lane_value<rk + 8 * rk>(f[ak][ak]), lane_value<8 * rk>(s));
+        const bool own = c == rk;
+        const T sc = own ? hh.inv : T(1); /* x 1 is exact */
+        sfor<ak + 1, RS>([&](auto a) { f[a][ak] = f[a][ak] * sc; });
+        f[ak][ak] = own ? ((r > rk) ? f[ak][ak] * hh.inv : ((r == rk) ? hh.rdiag : f[ak][ak]))
+                        : f[ak][ak];
+        return hh.tau;
+    };
+    T vv[RS];
+    auto bcast = [&](auto kc) {
+        constexpr int kk = kc;
+        constexpr int ak = kk / 8, rk = kk % 8;
+        sfor<ak + 1, RS>([&](auto a) { vv[a] = colbc<rk>(f[a][ak], raddr); });
+        const T v0 = colbc<rk>(f[ak][ak], raddr);
+        vv[ak] = (r > rk) ? v0 : ((r == rk) ? T(1) : T(0));
+    };
+    auto dot = [&](auto kc, auto bc, T tau) {
+        constexpr int kk = kc, b = bc;
+        constexpr int ak = kk / 8, rk = kk % 8;
+        T d;
+        if constexpr (b < CS) {
+            d = vv[ak] * f[ak][b];
+            sfor<ak + 1, RS>([&](auto a) { d = fmaT(vv[a], f[a][b], d); });
+        }
+        else {
+            d = vv[ak] * gb[ak];
+            sfor<ak + 1, RS>([&](auto a) { d = fmaT(vv[a], gb[a], d); });
+        }
+        d = row_sum(d);
+        if constexpr (b == CS) return d * tau;
+        return (b > ak || c > rk) ? d * tau : T(0);
+    };
+    auto upd = [&](auto kc, auto bc, T w) {
+        constexpr int kk = kc, b = bc;
+        constexpr int ak = kk / 8, rk = kk % 8;
+        if constexpr (b < CS) {
+            f[ak][b] = (r >= rk) ? fmaT(-vv[ak], w, f[ak][b]) : f[ak][b];
+            sfor<ak + 1, RS>([&](auto a) { f[a][b] = fmaT(-vv[a], w, f[a][b]); });
+        }
+        else {
+            gb[ak] = (r >= rk) ? fmaT(-vv[ak], w, gb[ak]) : gb[ak];
+            sfor<ak + 1, RS>([&](auto a) { gb[a] = fmaT(-vv[a], w, gb[a]); });
+        }
+    };
+    /* look-ahead, as in qr_grid.hip: past column n - 1 the generated
+     * reflector comes from a zero column (tau = 0) and is never stored */
+    T tau = T(0);
+    if (n > 0) {
+        tau = gen(std::integral_constant<int, 0>{});
+        bcast(std::integral_constant<int, 0>{});
+    }
+    sfor<0, KMAX>([&](auto kc) {
+        constexpr int kk = kc;
+        constexpr int bk = kk / 8, b1 = (kk + 1) / 8;
+        if (kk < n) {
+            if constexpr (kk + 1 < KMAX) {
+                constexpr auto B1 = std::integral_constant<int, b1>{};
+                upd(kc, B1, dot(kc, B1, tau));
+                const T tn = gen(std::integral_constant<int, kk + 1>{});
+                sfor<bk, CS + 1>([&](auto b) {
+                    if constexpr (b != b1) upd(kc, b, dot(kc, b, tau));
+                });
+                bcast(std::integral_constant<int, kk + 1>{});
+                tau = tn;
+            }
+            else {
+                sfor<bk, CS + 1>([&](auto b) { upd(kc, b, dot(kc, b, tau)); });
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    });
+}
+
+/* the first exact zero on R's diagonal, + 1 (0: none): R(l, l), l = c + 8b,
+ * sits in slot (b, b) of lane (c, c) = 9c */
+template <class T> __device__ __forceinline__ int first_zero_pivot(const T (&f)[RS][CS], int r, int c, int n)
+{
+    int bad = 0;
+    sfor<0, CS>([&](auto b) {
+        const unsigned long long z =
+            __builtin_amdgcn_ballot_w64(r == c && c + 8 * b < n && f[b][b] == T(0));
+        if (bad == 0 && z != 0ull) bad = (int)(__builtin_ctzll(z) / 9) + 8 * b + 1;
+    });
+    return bad;
+}
+
+/* R X = (Q^T B)(0:n): x_l = b_l / R(l, l), then b_i -= R(i, l) x_l for rows
+ * i < l (the reference's trsm order) */
+template <class T>
+__device__ __forceinline__ void back_substitute(const T (&f)[RS][CS], T (&gb)[RS], int r, int n, int raddr)
+{
+    sfor_down<0, KMAX>([&](auto lc) {
+        constexpr int l = lc;
+        constexpr int al = l / 8, rl = l % 8;
+        if (l < n) {
+            T rc[al + 1]; /* R(r + 8a, l) from column lane l % 8 */
+            sfor<0, al + 1>([&](auto a) { rc[a] = colbc<rl>(f[a][al], raddr); });
+            const T x = rowbc<rl>(gb[al]) / lane_value<rl + 8 * rl>(f[al][al]);
+            sfor<0, al>([&](auto a) { gb[a] = fmaT(-rc[a], x, gb[a]); });
+            gb[al] = (r < rl) ? fmaT(-rc[al], x, gb[al]) : ((r == rl) ? x : gb[al]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    });
+}
+
+/* a persistent grid over a class queue: each wavefront takes the next
+ * problem until the queue is empty; each problem column-major on its own */
+template <class T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
+qr_grid64_batch_kernel(BatchArgs<T> p)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 7, c = lane >> 3;
+    const int raddr = 4 * r; /* ds_bpermute byte address of lane (r, 0) */
+    const unsigned count = batch_count(p);
+    for (;;) {
+        unsigned kq = 0;
+        if (lane == 0) kq = atomicAdd(p.counter, 1u);
+        kq = __builtin_amdgcn_readfirstlane(kq);
+        if (kq >= count) return;
+        const BatchProblem d = p.desc[kq];
+        const int m = d.m, n = d.n, nrhs = d.nrhs;
+        T *acol = p.a + d.a_off + (long long)c * d.lda + r;
+        T *bcol = p.b + d.b_off + (long long)c * d.ldb + r;
+        const long long astep = 8ll * d.lda;
+
+        T f[RS][CS], gb[RS];
+        sfor<0, RS>([&](auto a) {
+            sfor<0, CS>([&](auto b) {
+                f[a][b] = (r + 8 * a < m && c + 8 * b < n) ? acol[b * astep + 8 * a] : T(0);
+            });
+            gb[a] = (r + 8 * a < m && c < nrhs) ? bcol[8 * a] : T(0);
+        });
+        factor<T>(f, gb, r, c, n, raddr);
+
+        T *acol2 = opaque(acol), *bcol2 = opaque(bcol);
+        const int r2 = opaque(r), c2 = opaque(c);
+        sfor<0, RS>([&](auto a) {
+            sfor<0, CS>([&](auto b) {
+                if (r2 + 8 * a < m && c2 + 8 * b < n) acol2[b * astep + 8 * a] = f[a][b];
+            });
+        });
+        const int bad = first_zero_pivot<T>(f, r, c, n);
+        if (lane == 0 && p.info && d.idx < p.ninfo) p.info[d.idx] = bad;
+        if (!(p.info && bad)) back_substitute<T>(f, gb, r, n, raddr);
+        sfor<0, RS>([&](auto a) {
+            if (r2 + 8 * a < m && c2 < nrhs) bcol2[8 * a] = gb[a];
+        });
+    }
+}
+
+} /* namespace qrgrid64 */
+
+/* batch kernel 11 */
+template <class T>
+hipError_t launch_qr_grid64_batch(const BatchArgs<T> &args, unsigned blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL((qrgrid64::qr_grid64_batch_kernel<T>), dim3(blocks), dim3(256), 0, s, args);
+    return hipGetLastError();
+}
+
+template <class T> hipError_t qr_grid64_batch_occupancy(int *blocks)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, qrgrid64::qr_grid64_batch_kernel<T>, 256, 0);
+}
+
+template hipError_t launch_qr_grid64_batch<double>(const BatchArgs<double> &, unsigned, hipStream_t);
+template hipError_t launch_qr_grid64_batch<float>(const BatchArgs<float> &, unsigned, hipStream_t);
+template hipError_t qr_grid64_batch_occupancy<double>(int *);
+template hipError_t qr_grid64_batch_occupancy<float>(int *);
+
+} /* namespace cbk */

@@ -3,9 +3,11 @@ oracle's batch gels on the same pools.
 
 Every kernel class is exercised: the team kernel (n + nrhs <= 16; rows <= 16,
 32, 64), the wavefront classes (n + nrhs <= 64; rows <= 32, 64: the lane-grid
-kernel for n <= 32 and nrhs <= 8, two problems per wavefront; otherwise two
+kernel for n <= 32 and nrhs <= 8, two problems per wavefront; the 8 x 8 lane
+grid for wider problems with nrhs <= 8, one per wavefront; otherwise two
 problems per wavefront when n + nrhs <= 32, one otherwise), the
-two-array wavefront kernel (n + nrhs > 64) and the general lane kernel
+two-array wavefront kernel (n + nrhs > 64; with nrhs <= 8 the 8 x 8 lane grid)
+and the general lane kernel
 (transposed, underdetermined, more than 64 rows, empty problems). The general kernel performs the oracle's
 operations and is compared bit for bit; the register kernels sum in another
 order and are held to the least-squares perturbation bound (batchcase.check_ls).
@@ -51,6 +53,9 @@ def _class_mix(r):
         (64, 32, 2), (64, 56, 8), (40, 33, 5), (63, 60, 4),      # wavefront, 64 rows
         (64, 20, 4), (50, 25, 7), (33, 17, 1), (30, 9, 8),       # two per wavefront (n + nrhs <= 32)
         (64, 64, 1), (64, 60, 8), (60, 58, 7), (64, 57, 8),      # two-array wavefront
+        (30, 12, 16), (32, 20, 10), (50, 20, 9),                 # nrhs > 8: two per wavefront
+        (60, 30, 12), (40, 16, 20), (32, 20, 14),                # nrhs > 8: one per wavefront
+        (64, 56, 10), (60, 50, 15),                              # nrhs > 8: two-array wavefront
         (70, 10, 2), (30, 40, 2), (5, 0, 2), (6, 3, 0),          # general
     ] + batchcase.uneven_shapes(r, 300)
 
