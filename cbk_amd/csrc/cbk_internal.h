@@ -262,11 +262,16 @@ template <class T> hipError_t qr_grid64_batch_occupancy(int *blocks);
 /* gels on compact operands through the wavefront kernel: the tall view p x q
  * (transposed: A^T), p <= 64, q + nrhs <= 64, nrhs >= 1, any V */
 bool qr_wave_supports(int p, int q, int nrhs);
-/* FP64 gels on a 4 x 8 lane grid per problem, two problems per wavefront
- * (qr_grid.hip): p <= 64, q <= 32, nrhs <= 8 */
+/* gels / geqrf on lane grids (qr_grid.hip, qr_grid64.hip): p <= 64, q <= 64,
+ * nrhs <= 8; two problems per wavefront on 4 x 8 lanes each for q <= 32, one
+ * on 8 x 8 lanes above */
 bool qr_grid_supports(int p, int q, int nrhs);
-hipError_t launch_qr_grid(int p, int q, int nrhs, bool transposed, Operand<double> A, double *tau,
-                          Operand<double> B, int V, int nm, int *info);
+template <class T>
+hipError_t launch_qr_grid(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
+                          Operand<T> B, int V, int nm, int *info);
+template <class T>
+hipError_t launch_qr_grid64(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
+                            Operand<T> B, int V, int nm, int *info);
 template <class T>
 hipError_t launch_qr_wave(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
                           Operand<T> B, int V, int nm, int *info);

@@ -126,10 +126,8 @@ hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operan
                 return launch_qr_wave<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
             break;
         case CBK_FAMILY_GRID:
-            if constexpr (sizeof(T) == 8) {
-                if (qr_grid_supports(p, q, nrhs))
-                    return launch_qr_grid(p, q, nrhs, !tall, A, tau, B, V, nm, info);
-            }
+            if (qr_grid_supports(p, q, nrhs))
+                return launch_qr_grid<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
             break;
         case CBK_FAMILY_LANE: return launch_gels_lane<T>(tran, m, n, nrhs, A, B, tau, V, nm, info);
         default: break;
@@ -140,15 +138,15 @@ hipError_t dispatch_gels(bool tran, int m, int n, int nrhs, Operand<T> A, Operan
      * column team for the other small shapes */
     if (current_policy() != CBK_POLICY_LANE && overdet && qr_reg_supports(p, q, fp64))
         return launch_qr_reg<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
+    /* (the lane grid first at more than 32 rows: sgels 64x32 nrhs 2 1518 us
+     * against the row team's 2470 us) */
     if (current_policy() == CBK_POLICY_AUTO && overdet && V >= 4 &&
-        qr_team_supports(p, q, nrhs, fp64))
+        qr_team_supports(p, q, nrhs, fp64) && !(p > 32 && qr_grid_supports(p, q, nrhs)))
         return launch_qr_team<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
     if (current_policy() == CBK_POLICY_AUTO && overdet && V >= 4 && qr_cols_supports(p, q, nrhs))
         return launch_qr_cols<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
-    if constexpr (sizeof(T) == 8) {
-        if (current_policy() != CBK_POLICY_LANE && overdet && qr_grid_supports(p, q, nrhs))
-            return launch_qr_grid(p, q, nrhs, !tall, A, tau, B, V, nm, info);
-    }
+    if (current_policy() != CBK_POLICY_LANE && overdet && qr_grid_supports(p, q, nrhs))
+        return launch_qr_grid<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
     if (current_policy() != CBK_POLICY_LANE && overdet && qr_wave_supports(p, q, nrhs))
         return launch_qr_wave<T>(p, q, nrhs, !tall, A, tau, B, V, nm, info);
     return launch_gels_lane<T>(tran, m, n, nrhs, A, B, tau, V, nm, info);
@@ -174,10 +172,8 @@ template <class T> hipError_t dispatch_geqrf(int m, int n, Operand<T> A, T *tau,
                 return launch_qr_cols<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
             break;
         case CBK_FAMILY_GRID:
-            if constexpr (sizeof(T) == 8) {
-                if (qr_grid_supports(m, n, 0))
-                    return launch_qr_grid(m, n, 0, false, A, tau, none, V, nm, nullptr);
-            }
+            if (qr_grid_supports(m, n, 0))
+                return launch_qr_grid<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
             break;
         case CBK_FAMILY_LANE: return launch_geqrf_lane<T>(m, n, A, tau, V, nm);
         default: break;
@@ -185,15 +181,13 @@ template <class T> hipError_t dispatch_geqrf(int m, int n, Operand<T> A, T *tau,
     }
     if (current_policy() != CBK_POLICY_LANE && tall && qr_reg_supports(m, n, fp64))
         return launch_qr_reg<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
-    if (current_policy() == CBK_POLICY_AUTO && tall && V >= 4 &&
-        qr_team_supports(m, n, 0, fp64))
+    if (current_policy() == CBK_POLICY_AUTO && tall && V >= 4 && qr_team_supports(m, n, 0, fp64) &&
+        !(m > 32 && qr_grid_supports(m, n, 0)))
         return launch_qr_team<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
     if (current_policy() == CBK_POLICY_AUTO && tall && V >= 4 && qr_cols_supports(m, n, 0))
         return launch_qr_cols<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
-    if constexpr (sizeof(T) == 8) {
-        if (current_policy() != CBK_POLICY_LANE && tall && qr_grid_supports(m, n, 0))
-            return launch_qr_grid(m, n, 0, false, A, tau, none, V, nm, nullptr);
-    }
+    if (current_policy() != CBK_POLICY_LANE && tall && qr_grid_supports(m, n, 0))
+        return launch_qr_grid<T>(m, n, 0, false, A, tau, none, V, nm, nullptr);
     return launch_geqrf_lane<T>(m, n, A, tau, V, nm);
 }
 

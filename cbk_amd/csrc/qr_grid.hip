@@ -51,8 +51,8 @@ using quad::qx;
 constexpr int CSA = 4;  /* A column slots: columns c + 8b */
 constexpr int KMAX = 32;
 
-struct GridArgs {
-    double *a, *b, *tau;
+template <class T> struct GridArgs {
+    T *a, *b, *tau;
     int *info;
     int a_rs, a_cs, b_rs, b_cs; /* element strides, V folded in */
     long long a_gs, b_gs;
@@ -246,11 +246,13 @@ __device__ __forceinline__ void back_substitute(const T (&f)[RS][CSA], T (&gb)[R
     });
 }
 
-/* compact FP64 operands, two consecutive slots per wavefront */
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
-qr_grid_kernel(GridArgs p)
+/* compact operands, two consecutive slots per wavefront */
+template <class T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, sizeof(T) == 8 ? 2 : 4))) void
+qr_grid_kernel(GridArgs<T> p)
 {
     constexpr int RS = 16;
+    constexpr unsigned SZ = sizeof(T);
     const int lane = threadIdx.x & 63;
     const int h = lane >> 5, c = (lane >> 2) & 7, r = lane & 3;
     /* blocks in XCD order: the 32 problems of 4 consecutive logical blocks
@@ -268,12 +270,12 @@ qr_grid_kernel(GridArgs p)
     const long long groups = p.count / p.V;
     const long long g = live ? kq / p.V : w0, v = live ? kq % p.V : 0;
     const __amdgpu_buffer_rsrc_t ra =
-        team::group_rsrc(p.a + w0 * p.a_gs, team::group_range(groups - w0, p.a_gs, 8));
+        team::group_rsrc(p.a + w0 * p.a_gs, team::group_range(groups - w0, p.a_gs, SZ));
     const __amdgpu_buffer_rsrc_t rb =
-        team::group_rsrc(p.b + w0 * p.b_gs, team::group_range(groups - w0, p.b_gs, 8));
-    const unsigned ars = 8u * p.a_rs, acs = 8u * p.a_cs, brs = 8u * p.b_rs, bcs = 8u * p.b_cs;
-    const unsigned aoff = (unsigned)(((g - w0) * p.a_gs + v) * 8) + r * ars + c * acs;
-    const unsigned boff = (unsigned)(((g - w0) * p.b_gs + v) * 8) + r * brs + c * bcs;
+        team::group_rsrc(p.b + w0 * p.b_gs, team::group_range(groups - w0, p.b_gs, SZ));
+    const unsigned ars = SZ * p.a_rs, acs = SZ * p.a_cs, brs = SZ * p.b_rs, bcs = SZ * p.b_cs;
+    const unsigned aoff = (unsigned)(((g - w0) * p.a_gs + v) * SZ) + r * ars + c * acs;
+    const unsigned boff = (unsigned)(((g - w0) * p.b_gs + v) * SZ) + r * brs + c * bcs;
     /* the masked offsets of element slot (a, b) from lane offsets lo (A) and
      * lb (B) and row/column lanes rr, cc; the stores at the end pass their
      * operands through opaque() so that the compiler recomputes these few
@@ -285,15 +287,15 @@ qr_grid_kernel(GridArgs p)
         return (live && rr + 4 * a < m && cc < nrhs) ? lo : team::kOOB;
     };
 
-    double f[RS][CSA], gb[RS];
+    T f[RS][CSA], gb[RS];
     sfor<0, RS>([&](auto a) {
         sfor<0, CSA>([&](auto b) {
-            f[a][b] = team::bload(0.0, ra, a_at(aoff, r, c, a, b), a * 4 * ars + b * 8 * acs);
+            f[a][b] = team::bload(T(0), ra, a_at(aoff, r, c, a, b), a * 4 * ars + b * 8 * acs);
         });
-        gb[a] = team::bload(0.0, rb, b_at(boff, r, c, a), a * 4 * brs);
+        gb[a] = team::bload(T(0), rb, b_at(boff, r, c, a), a * 4 * brs);
     });
-    double tauv[CSA] = {0.0, 0.0, 0.0, 0.0};
-    factor<double, RS>(f, gb, tauv, r, c, n);
+    T tauv[CSA] = {T(0), T(0), T(0), T(0)};
+    factor<T, RS>(f, gb, tauv, r, c, n);
 
     /* A: R and the reflector bodies; tau (reflector c + 8b of the lane's
      * column lane, at ((g * n + kk) * V + v)) */
@@ -305,23 +307,23 @@ qr_grid_kernel(GridArgs p)
         });
     });
     if (p.tau) {
-        const unsigned tV = 8u * p.V;
+        const unsigned tV = SZ * p.V;
         const __amdgpu_buffer_rsrc_t rt =
             team::group_rsrc(p.tau + w0 * (long long)n * p.V,
-                             team::group_range(groups - w0, (size_t)n * p.V, 8));
-        const unsigned toff = (unsigned)(((g - w0) * n * p.V + v) * 8) + c * tV;
+                             team::group_range(groups - w0, (size_t)n * p.V, SZ));
+        const unsigned toff = (unsigned)(((g - w0) * n * p.V + v) * SZ) + c * tV;
         sfor<0, CSA>([&](auto b) {
             const unsigned o = (live && r == 0 && c + 8 * b < n) ? toff : team::kOOB;
             team::bstore(tauv[b], rt, o, b * 8 * tV);
         });
     }
 
-    const int bad = first_zero_pivot<double, RS>(f, r, c, n);
+    const int bad = first_zero_pivot<T, RS>(f, r, c, n);
     if (live && lane % 32 == 0 && p.info && kq < p.ninfo) p.info[kq] = bad;
     /* LAPACK dgels stops at a singular R (B keeps Q^T B); without info the
      * substitution runs as the reference's gels_compact. bad is uniform in
      * the half. */
-    if (nrhs > 0 && !(p.info && bad)) back_substitute<double, RS>(f, gb, r, n);
+    if (nrhs > 0 && !(p.info && bad)) back_substitute<T, RS>(f, gb, r, n);
     sfor<0, RS>([&](auto a) { team::bstore(gb[a], rb, b_at(boff2, r2, c2, a), a * 4 * brs); });
 }
 
@@ -378,16 +380,20 @@ qr_grid_batch_kernel(BatchArgs<T> p)
 
 } /* namespace qrgrid */
 
-/* gels with nrhs <= 8, or geqrf (nrhs = 0), of a tall p x q view */
+/* gels with nrhs <= 8, or geqrf (nrhs = 0), of a tall p x q view: two
+ * problems per wavefront for q <= 32, the 8 x 8 lane grid (qr_grid64.hip)
+ * above */
 bool qr_grid_supports(int p, int q, int nrhs)
 {
-    return p <= 64 && q >= 1 && q <= 32 && q <= p && nrhs >= 0 && nrhs <= 8;
+    return p <= 64 && q >= 1 && q <= 64 && q <= p && nrhs >= 0 && nrhs <= 8;
 }
 
-hipError_t launch_qr_grid(int p, int q, int nrhs, bool transposed, Operand<double> A, double *tau,
-                          Operand<double> B, int V, int nm, int *info)
+template <class T>
+hipError_t launch_qr_grid(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
+                          Operand<T> B, int V, int nm, int *info)
 {
-    qrgrid::GridArgs args;
+    if (q > 32) return launch_qr_grid64<T>(p, q, nrhs, transposed, A, tau, B, V, nm, info);
+    qrgrid::GridArgs<T> args;
     int si = A.row ? A.ld : 1, sj = A.row ? 1 : A.ld;
     if (transposed) {
         const int tmp = si;
@@ -412,9 +418,14 @@ hipError_t launch_qr_grid(int p, int q, int nrhs, bool transposed, Operand<doubl
     args.count = (long long)((nm + V - 1) / V) * V; /* padded slots too, as the reference */
     const long long waves = (args.count + 1) / 2;
     const unsigned blocks = team::xcd_grid((unsigned)((waves + 3) / 4));
-    hipLaunchKernelGGL(qrgrid::qr_grid_kernel, dim3(blocks), dim3(256), 0, current_stream(), args);
+    hipLaunchKernelGGL((qrgrid::qr_grid_kernel<T>), dim3(blocks), dim3(256), 0, current_stream(), args);
     return hipGetLastError();
 }
+
+template hipError_t launch_qr_grid<double>(int, int, int, bool, Operand<double>, double *,
+                                           Operand<double>, int, int, int *);
+template hipError_t launch_qr_grid<float>(int, int, int, bool, Operand<float>, float *,
+                                          Operand<float>, int, int, int *);
 
 /* batch kernels 9 (rows <= 32) and 10 (rows <= 64) */
 template <class T>

@@ -101,9 +101,10 @@ template <class T> __device__ __forceinline__ T opaque(T x)
 }
 
 /* QR of the problem's n columns (rows past the problem zero), Q^T B in gb as
- * it goes */
+ * it goes; tauv[b] = tau of reflector c + 8b */
 template <class T>
-__device__ __forceinline__ void factor(T (&f)[RS][CS], T (&gb)[RS], int r, int c, int n, int raddr)
+__device__ __forceinline__ void factor(T (&f)[RS][CS], T (&gb)[RS], T (&tauv)[CS], int r, int c, int n,
+                                       int raddr)
 {
     auto gen = [&](auto kc) {
         constexpr int kk = kc;
@@ -124,6 +125,7 @@ __device__ __forceinline__ void factor(T (&f)[RS][CS], T (&gb)[RS], int r, int c
         sfor<ak + 1, RS>([&](auto a) { f[a][ak] = f[a][ak] * sc; });
         f[ak][ak] = own ? ((r > rk) ? f[ak][ak] * hh.inv : ((r == rk) ? hh.rdiag : f[ak][ak]))
                         : f[ak][ak];
+        tauv[ak] = own ? hh.tau : tauv[ak];
         return hh.tau;
     };
     T vv[RS];
@@ -251,7 +253,9 @@ qr_grid64_batch_kernel(BatchArgs<T> p)
             });
             gb[a] = (r + 8 * a < m && c < nrhs) ? bcol[8 * a] : T(0);
         });
-        factor<T>(f, gb, r, c, n, raddr);
+        T tauv[CS];
+        sfor<0, CS>([&](auto b) { tauv[b] = T(0); });
+        factor<T>(f, gb, tauv, r, c, n, raddr);
 
         T *acol2 = opaque(acol), *bcol2 = opaque(bcol);
         const int r2 = opaque(r), c2 = opaque(c);
@@ -269,7 +273,117 @@ qr_grid64_batch_kernel(BatchArgs<T> p)
     }
 }
 
+/* compact operands (gels_compact / geqrf_compact with 32 < q <= 64): one
+ * problem per wavefront, the wavefronts of 4 consecutive logical blocks (16
+ * consecutive slots, a 128-byte run of each element at V >= 16 in FP64) on
+ * one XCD; buffer addressing as in qr_grid.hip */
+template <class T> struct Grid64Args {
+    T *a, *b, *tau;
+    int *info;
+    int a_rs, a_cs, b_rs, b_cs; /* element strides, V folded in */
+    long long a_gs, b_gs;
+    int V, m, n, nrhs, ninfo;
+    long long count; /* slots, padded ones included */
+};
+
+template <class T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
+qr_grid64_kernel(Grid64Args<T> p)
+{
+    constexpr unsigned SZ = sizeof(T);
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 7, c = lane >> 3;
+    const int raddr = 4 * r;
+    const long long kq = (long long)team::xcd_block(blockIdx.x) * 4 + team::wave_in_block();
+    if (kq >= p.count) return; /* whole wavefront */
+    const int m = p.m, n = p.n, nrhs = p.nrhs;
+    const long long g = kq / p.V, v = kq % p.V;
+    const long long groups = p.count / p.V;
+    const __amdgpu_buffer_rsrc_t ra =
+        team::group_rsrc(p.a + g * p.a_gs, team::group_range(groups - g, p.a_gs, SZ));
+    const __amdgpu_buffer_rsrc_t rb =
+        team::group_rsrc(p.b + g * p.b_gs, team::group_range(groups - g, p.b_gs, SZ));
+    const unsigned ars = SZ * p.a_rs, acs = SZ * p.a_cs, brs = SZ * p.b_rs, bcs = SZ * p.b_cs;
+    const unsigned aoff = (unsigned)(v * SZ) + r * ars + c * acs;
+    const unsigned boff = (unsigned)(v * SZ) + r * brs + c * bcs;
+    auto a_at = [&](unsigned lo, int rr, int cc, int a, int b) {
+        return (rr + 8 * a < m && cc + 8 * b < n) ? lo : team::kOOB;
+    };
+    auto b_at = [&](unsigned lo, int rr, int cc, int a) {
+        return (rr + 8 * a < m && cc < nrhs) ? lo : team::kOOB;
+    };
+    T f[RS][CS], gb[RS];
+    sfor<0, RS>([&](auto a) {
+        sfor<0, CS>([&](auto b) {
+            f[a][b] = team::bload(T(0), ra, a_at(aoff, r, c, a, b), a * 8 * ars + b * 8 * acs);
+        });
+        gb[a] = team::bload(T(0), rb, b_at(boff, r, c, a), a * 8 * brs);
+    });
+    T tauv[CS];
+    sfor<0, CS>([&](auto b) { tauv[b] = T(0); });
+    factor<T>(f, gb, tauv, r, c, n, raddr);
+
+    const unsigned aoff2 = opaque(aoff), boff2 = opaque(boff);
+    const int r2 = opaque(r), c2 = opaque(c);
+    sfor<0, RS>([&](auto a) {
+        sfor<0, CS>([&](auto b) {
+            team::bstore(f[a][b], ra, a_at(aoff2, r2, c2, a, b), a * 8 * ars + b * 8 * acs);
+        });
+    });
+    if (p.tau) {
+        /* tau of reflector kk at ((g * n + kk) * V + v) */
+        const unsigned tV = SZ * p.V;
+        const __amdgpu_buffer_rsrc_t rt = team::group_rsrc(
+            p.tau + g * (long long)n * p.V, team::group_range(groups - g, (size_t)n * p.V, SZ));
+        const unsigned toff = (unsigned)(v * SZ) + c * tV;
+        sfor<0, CS>([&](auto b) {
+            team::bstore(tauv[b], rt, (r == 0 && c + 8 * b < n) ? toff : team::kOOB, b * 8 * tV);
+        });
+    }
+    const int bad = first_zero_pivot<T>(f, r, c, n);
+    if (lane == 0 && p.info && kq < p.ninfo) p.info[kq] = bad;
+    if (nrhs > 0 && !(p.info && bad)) back_substitute<T>(f, gb, r, n, raddr);
+    sfor<0, RS>([&](auto a) { team::bstore(gb[a], rb, b_at(boff2, r2, c2, a), a * 8 * brs); });
+}
+
 } /* namespace qrgrid64 */
+
+template <class T>
+hipError_t launch_qr_grid64(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
+                            Operand<T> B, int V, int nm, int *info)
+{
+    qrgrid64::Grid64Args<T> args;
+    int si = A.row ? A.ld : 1, sj = A.row ? 1 : A.ld;
+    if (transposed) {
+        const int tmp = si;
+        si = sj;
+        sj = tmp;
+    }
+    args.a = A.p;
+    args.b = B.p;
+    args.tau = tau;
+    args.info = info;
+    args.a_rs = si * V;
+    args.a_cs = sj * V;
+    args.b_rs = (B.row ? B.ld : 1) * V;
+    args.b_cs = (B.row ? 1 : B.ld) * V;
+    args.a_gs = (long long)A.gstride;
+    args.b_gs = (long long)B.gstride;
+    args.V = V;
+    args.m = p;
+    args.n = q;
+    args.nrhs = nrhs;
+    args.ninfo = nm;
+    args.count = (long long)((nm + V - 1) / V) * V; /* padded slots too, as the reference */
+    const unsigned blocks = team::xcd_grid((unsigned)((args.count + 3) / 4));
+    hipLaunchKernelGGL((qrgrid64::qr_grid64_kernel<T>), dim3(blocks), dim3(256), 0, current_stream(), args);
+    return hipGetLastError();
+}
+
+template hipError_t launch_qr_grid64<double>(int, int, int, bool, Operand<double>, double *,
+                                             Operand<double>, int, int, int *);
+template hipError_t launch_qr_grid64<float>(int, int, int, bool, Operand<float>, float *,
+                                            Operand<float>, int, int, int *);
 
 /* batch kernel 11 */
 template <class T>

@@ -120,7 +120,7 @@ std::vector<int> candidates(const Shape &s, int V)
         if (V >= 4 && qr_team_supports(s.p, s.q, r, s.fp64)) f.push_back(CBK_FAMILY_TEAM);
         if (V >= 4 && qr_cols_supports(s.p, s.q, r)) f.push_back(CBK_FAMILY_COLUMNS);
         if (s.cls == kTuneGels && qr_wave_supports(s.p, s.q, r)) f.push_back(CBK_FAMILY_WAVE);
-        if (s.fp64 && qr_grid_supports(s.p, s.q, r)) f.push_back(CBK_FAMILY_GRID);
+        if (qr_grid_supports(s.p, s.q, r)) f.push_back(CBK_FAMILY_GRID);
         break;
     }
     }

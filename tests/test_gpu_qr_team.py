@@ -1,6 +1,7 @@
 """GPU parity of the QR team kernel (geqrf, overdetermined gels at the shapes it
-serves: 16 x 8 and 32 x 16 in both precisions, 64 x 32 in FP32) against the
-oracle. The team sums reflector norms and dot products as a butterfly, so the
+serves: 16 x 8 and 32 x 16 in both precisions, 64 x 32 in FP32; selected
+through the kernel-family table, since the register and lane-grid kernels come
+first at some of these shapes) against the oracle. The team sums reflector norms and dot products as a butterfly, so the
 comparison is to rounding:
   - R, the reflectors and tau: |gpu - oracle| <= 20 p eps ||A||_max (tau: 20 p eps);
   - X: |x_gpu - x_oracle|_inf <= 20 p eps (kappa + kappa^2 ||r|| / (||A|| ||x||)) ||x||_inf,
@@ -25,6 +26,17 @@ def _gpu():
     return gpuutil
 
 
+def _team_run(routine, p, m, n, nrhs, V, c, **kw):
+    """the case on the GPU with the team family installed for its shape"""
+    import cbk_amd
+    F = cbk_amd.FAMILY
+    assert cbk_amd.set_kernel_family(p, routine, m, n, nrhs, V, F["team"]) == 0
+    try:
+        return cases.run(_gpu().gpu_call, c, **kw)
+    finally:
+        cbk_amd.set_kernel_family(p, routine, m, n, nrhs, V, F["default"])
+
+
 @pytest.mark.parametrize("p,m,n", SHAPES)
 @pytest.mark.parametrize("layout,trans,V", [("C", "N", 64), ("R", "N", 64), ("C", "T", 64),
                                            ("C", "N", 32)])
@@ -37,7 +49,7 @@ def test_qr_team_gels(p, m, n, layout, trans, V):
         c = cases.make_case("gels", p, {"layout": layout, "trans": trans},
                             {"m": am, "n": an, "nrhs": nrhs}, V, nm, seed=3000 + m + nrhs)
         so, (ao, bo, to) = cases.run(cbkt.oracle().call, c)
-        sg, (ag, bg, tg) = cases.run(_gpu().gpu_call, c)
+        sg, (ag, bg, tg) = _team_run("gels", p, m, n, nrhs, V, c)
         assert so == sg == 0
         eps = float(np.finfo(cbkt.DT[p]).eps)
         row = layout == "R"
@@ -73,7 +85,7 @@ def test_qr_team_geqrf(p, m, n):
     V, nm = 16, 50
     c = cases.make_case("geqrf", p, {"layout": "C"}, {"m": m, "n": n}, V, nm, seed=77 + m)
     so, (ao, to) = cases.run(cbkt.oracle().call, c)
-    sg, (ag, tg) = cases.run(_gpu().gpu_call, c)
+    sg, (ag, tg) = _team_run("geqrf", p, m, n, 0, V, c)
     assert so == sg == 0
     eps = float(np.finfo(cbkt.DT[p]).eps)
     A = cbkt.unpack(c["args"][3], nm, m, n, V).astype(np.float64)
@@ -97,7 +109,7 @@ def test_qr_team_gels_info():
     info_o = np.zeros(nm, np.int32)
     info_g = np.zeros(nm, np.int32)
     so, (ao, bo, to) = cases.run(cbkt.oracle().call, c, info=info_o)
-    sg, (ag, bg, tg) = cases.run(_gpu().gpu_call, c, info=info_g)
+    sg, (ag, bg, tg) = _team_run("gels", "d", m, n, 2, V, c, info=info_g)
     assert so == sg == 0
     assert info_o[10] == 4 and info_o[40] == 1 and np.count_nonzero(info_o) == 2
     assert np.array_equal(info_o, info_g)

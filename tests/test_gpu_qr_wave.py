@@ -1,10 +1,12 @@
-"""GPU parity of gels_compact through the wavefront kernel (one matrix per
-wavefront, lane c owning column c of [A | B]; the tall view p x q with
-p <= 64 and q + nrhs <= 64 where neither QR team kernel applies: more than 32
-rows, more than 16 columns of [A | B], or V < 4)
-against the oracle. Sums are in another order than the reference's (four
-partial sums), so the comparison is to the least-squares perturbation bound,
-as for the team kernel:
+"""GPU parity of gels_compact on the shapes where neither QR team kernel
+applies (the tall view p x q with p <= 64 and q + nrhs <= 64: more than 32
+rows, more than 16 columns of [A | B], or V < 4), through the default dispatch
+(the lane-grid kernels: two problems per wavefront on 4 x 8 lanes each for
+q <= 32, one on 8 x 8 lanes above, nrhs <= 8) and through the wavefront kernel
+(one matrix per wavefront, lane c owning column c of [A | B]) forced by the
+kernel-family table, against the oracle. Sums are in another order than the
+reference's (per-lane and cross-lane partial sums), so the comparison is to
+the least-squares perturbation bound, as for the team kernel:
   - R, the reflectors and tau: |gpu - oracle| <= 20 p eps ||A||_max (tau: 20 p eps);
   - X: |x_gpu - x_oracle|_inf <= 20 p eps (kappa + kappa^2 ||r|| / (||A|| ||x||)) ||x||_inf;
   - the rows of Q^T B below q: 20 p eps kappa ||b||;
@@ -28,16 +30,23 @@ def _gpu():
     return gpuutil
 
 
+@pytest.mark.parametrize("family", ["default", "wave"])
 @pytest.mark.parametrize("p,m,n,nrhs", SHAPES)
 @pytest.mark.parametrize("layout,trans,V,nm", [("C", "N", 64, 130), ("R", "N", 16, 37),
                                                ("C", "T", 2, 9), ("C", "N", 32, 71)])
-def test_qr_wave_gels(p, m, n, nrhs, layout, trans, V, nm):
+def test_qr_wave_gels(p, m, n, nrhs, layout, trans, V, nm, family):
+    import cbk_amd
     am, an = (m, n) if trans == "N" else (n, m)
-    print(f"case gels-wave {p} {am}x{an} nrhs={nrhs} {layout}{trans} V={V} nm={nm}", flush=True)
+    print(f"case gels-{family} {p} {am}x{an} nrhs={nrhs} {layout}{trans} V={V} nm={nm}", flush=True)
     c = cases.make_case("gels", p, {"layout": layout, "trans": trans},
                         {"m": am, "n": an, "nrhs": nrhs}, V, nm, seed=5000 + m + n + nrhs)
     so, (ao, bo, to) = cases.run(cbkt.oracle().call, c)
-    sg, (ag, bg, tg) = cases.run(_gpu().gpu_call, c)
+    F = cbk_amd.FAMILY
+    assert cbk_amd.set_kernel_family(p, "gels", m, n, nrhs, V, F[family]) == 0
+    try:
+        sg, (ag, bg, tg) = cases.run(_gpu().gpu_call, c)
+    finally:
+        cbk_amd.set_kernel_family(p, "gels", m, n, nrhs, V, F["default"])
     assert so == sg == 0
     eps = float(np.finfo(cbkt.DT[p]).eps)
     row = layout == "R"
@@ -72,12 +81,13 @@ def test_qr_wave_gels(p, m, n, nrhs, layout, trans, V, nm):
                                cbkt.unpack(ao, nm, am, an, V, ld=c["args"][6], row=row, all_slots=True)[nm:])
 
 
-def test_qr_wave_gels_info():
-    V, nm, m, n = 64, 70, 64, 32
+@pytest.mark.parametrize("n,zcol", [(32, 17), (48, 40)])
+def test_qr_wave_gels_info(n, zcol):
+    V, nm, m = 64, 70, 64
     c = cases.make_case("gels", "d", {"layout": "C", "trans": "N"}, {"m": m, "n": n, "nrhs": 2},
                         V, nm, seed=77)
     A = cbkt.unpack(c["args"][5], nm, m, n, V)
-    A[10, :, 17] = 0.0
+    A[10, :, zcol] = 0.0
     A[40, :, 0] = 0.0
     c["args"][5] = cbkt.pack(A, V)
     info_o = np.zeros(nm, np.int32)
@@ -85,7 +95,7 @@ def test_qr_wave_gels_info():
     so, (ao, bo, to) = cases.run(cbkt.oracle().call, c, info=info_o)
     sg, (ag, bg, tg) = cases.run(_gpu().gpu_call, c, info=info_g)
     assert so == sg == 0
-    assert info_o[10] == 18 and info_o[40] == 1 and np.count_nonzero(info_o) == 2
+    assert info_o[10] == zcol + 1 and info_o[40] == 1 and np.count_nonzero(info_o) == 2
     assert np.array_equal(info_o, info_g)
     Bo = cbkt.unpack(bo, nm, m, 2, V)
     Bg = cbkt.unpack(bg, nm, m, 2, V)
