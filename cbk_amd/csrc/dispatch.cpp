@@ -36,10 +36,14 @@ hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand
      * whole triangle in one lane: ssysvnp nrhs = 24 450 vs 523 us), else the
      * whole triangle in one lane's registers where it fits (faster than the
      * team kernel at its orders 16 and 24) */
-    if (fit && current_policy() == CBK_POLICY_AUTO && quad_sym_supports(n, fp64))
+    if (fit && current_policy() == CBK_POLICY_AUTO && (n == 24 || n == 32) && quad_sym_supports(n, fp64))
         return launch_quad_sym<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     if (fit && current_policy() != CBK_POLICY_LANE && reg_chol_supports(n, fp64))
         return launch_reg_chol<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+    /* the other orders up to 32 on the next quad instance, as diag(A, I)
+     * (dposv n = 20 / 28: 1049 / 2509 us on the lane kernel) */
+    if (fit && current_policy() == CBK_POLICY_AUTO && quad_sym_supports(n, fp64))
+        return launch_quad_sym<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     if (fit && current_policy() == CBK_POLICY_AUTO && sym_team_supports(n) && V >= 4)
         return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);

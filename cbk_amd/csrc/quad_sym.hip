@@ -12,10 +12,12 @@ extern template hipError_t launch_n<float, 24>(bool, const QuadSymArgs<float> &)
 extern template hipError_t launch_n<float, 32>(bool, const QuadSymArgs<float> &);
 } /* namespace quadsym */
 
+/* the instances of order 24 and 32; an order between 16 and 32 runs on the
+ * next one as diag(A, I) */
 bool quad_sym_supports(int n, bool fp64)
 {
     (void)fp64;
-    return n == 24 || n == 32;
+    return n > 16 && n <= 32;
 }
 
 template <class T>
@@ -45,14 +47,9 @@ hipError_t launch_quad_sym(bool ldl, bool factor, bool up, int n, int nrhs, Oper
     args.nrhs = nrhs;
     args.info = info;
     args.factor = factor;
-    if constexpr (sizeof(T) == 8) {
-        if (n == 24) return quadsym::launch_n<T, 24>(ldl, args);
-        if (n == 32) return quadsym::launch_n<T, 32>(ldl, args);
-    }
-    else {
-        if (n == 24) return quadsym::launch_n<T, 24>(ldl, args);
-        if (n == 32) return quadsym::launch_n<T, 32>(ldl, args);
-    }
+    args.n = n;
+    if (n > 16 && n <= 24) return quadsym::launch_n<T, 24>(ldl, args);
+    if (n > 24 && n <= 32) return quadsym::launch_n<T, 32>(ldl, args);
     return hipErrorInvalidValue;
 }
 

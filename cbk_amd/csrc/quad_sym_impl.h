@@ -8,7 +8,11 @@
  * blocks of NRB right-hand sides. Rows past a lane's diagonal and masked
  * stores use an out-of-range offset (zero load / dropped store), so no load
  * or store branches. With info, a matrix whose factorization failed keeps its
- * B (LAPACK dposv skips dpotrs).
+ * B (LAPACK dposv skips dpotrs). An order n below the instance's N runs as
+ * diag(A, I): rows and columns past n read as zero with 1 on the diagonal and
+ * are never stored, B's rows past n as zero; every element of the leading
+ * n x n block gets exactly the operations of the order-n factorization, so
+ * the factor stays bit-identical to the reference's.
  */
 #ifndef CBK_QUAD_SYM_IMPL_H
 #define CBK_QUAD_SYM_IMPL_H
@@ -41,6 +45,7 @@ template <class T> struct QuadSymArgs {
     int q0;      /* first right-hand side of this launch (one-block instances) */
     const int *skip; /* one-block launches after the first: the info the first wrote */
     const int *only; /* if set, only the matrices with only[mat] < 0 are touched */
+    int n;           /* the order, n <= N: rows and columns past it read as the identity */
 };
 
 /* resident wavefronts per SIMD the kernel is built for: the FP64 order 32
@@ -89,8 +94,10 @@ quad_sym_kernel(QuadSymArgs<T> p)
     const unsigned aslot = pl.slot(p.a_gs, p.V, SZ);
     const unsigned a_col = (unsigned)p.a_sj * VS;
     unsigned a_row[K]; /* byte offset of (row t + 4k, column 0) */
+    const int n = p.n;
     sfor<0, K>([&](auto k) {
-        a_row[k] = (aslot == kOOB) ? kOOB : aslot + (unsigned)((t + 4 * k) * p.a_si) * VS;
+        a_row[k] = (aslot == kOOB || t + 4 * k >= n) ? kOOB
+                                                     : aslot + (unsigned)((t + 4 * k) * p.a_si) * VS;
     });
     T *L = lds + (threadIdx.x >> 2) * stage_elems<N>();
 
@@ -100,7 +107,8 @@ quad_sym_kernel(QuadSymArgs<T> p)
     const unsigned b_col = (unsigned)p.b_sj * VS;
     unsigned b_row[K];
     sfor<0, K>([&](auto k) {
-        b_row[k] = (bslot == kOOB) ? kOOB : bslot + (unsigned)((t + 4 * k) * p.b_si) * VS;
+        b_row[k] = (bslot == kOOB || t + 4 * k >= n) ? kOOB
+                                                     : bslot + (unsigned)((t + 4 * k) * p.b_si) * VS;
     });
     auto load_b = [&](T(&b)[K][NRB], int q0) {
         sfor<0, NRB>([&](auto q) {
@@ -124,9 +132,11 @@ quad_sym_kernel(QuadSymArgs<T> p)
             constexpr int P = Pc;
             sfor<8 * P, 8 * P + 8>([&](auto c) {
                 sfor<c / 4, K>([&](auto k) {
-                    a[k][c] = bload(T(), ra, (c < 4 * k || c <= t + 4 * k) ? a_row[k] : kOOB,
+                    a[k][c] = bload(T(), ra, (c < n && (c < 4 * k || c <= t + 4 * k)) ? a_row[k] : kOOB,
                                     c * a_col);
                 });
+                /* the identity past n: 1 on the diagonal (row t + 4k = c) */
+                a[c / 4][c] = (c >= n && c == t + 4 * (c / 4)) ? T(1) : a[c / 4][c];
             });
             if constexpr (P == N / 8 - 1 && prefetch_b<T, N>()) {
                 if (p.nrhs > 0) load_b(b, qfirst);
@@ -136,7 +146,8 @@ quad_sym_kernel(QuadSymArgs<T> p)
             constexpr int P = Pc;
             sfor<8 * P, 8 * P + 8>([&](auto c) {
                 sfor<c / 4, K>([&](auto k) {
-                    bstore(a[k][c], ra, (c < 4 * k || c < t + 4 * k) ? a_row[k] : kOOB, c * a_col);
+                    bstore(a[k][c], ra, (c < n && (c < 4 * k || c < t + 4 * k)) ? a_row[k] : kOOB,
+                           c * a_col);
                 });
             });
             sfor<2 * P, 2 * P + 2>([&](auto k) {
@@ -154,11 +165,11 @@ quad_sym_kernel(QuadSymArgs<T> p)
     else {
         sfor<0, K>([&](auto k) {
             sfor<0, 4 * k + 4>([&](auto c) {
-                a[k][c] = bload(T(), ra, (c < 4 * k || c < t + 4 * k) ? a_row[k] : kOOB,
+                a[k][c] = bload(T(), ra, (c < n && (c < 4 * k || c < t + 4 * k)) ? a_row[k] : kOOB,
                                 c * a_col);
             });
             const unsigned doff = (a_row[k] == kOOB) ? kOOB : a_row[k] + (unsigned)(t + 4 * k) * a_col;
-            dg[k] = bload(T(), ra, doff, 0);
+            dg[k] = (t + 4 * k >= n) ? T(1) : bload(T(), ra, doff, 0);
         });
         if (p.nrhs > 0) load_b(b, qfirst);
     }
