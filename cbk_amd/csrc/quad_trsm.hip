@@ -4,9 +4,12 @@
 
 namespace cbk {
 
+/* the FP64 instance of order 24 and the FP32 one of order 32; the orders
+ * above the register kernel's (FP64 16, FP32 24) run on them as
+ * diag(op(A), I) */
 bool quad_trsm_supports(bool left, bool up, bool tran, int m, bool fp64)
 {
-    return left && up == tran && (fp64 ? m == 24 : m == 32);
+    return left && up == tran && (fp64 ? (m > 16 && m <= 24) : (m > 24 && m <= 32));
 }
 
 template <class T>
@@ -34,11 +37,12 @@ hipError_t launch_quad_trsm(bool tran, bool unit, int m, int n, T alpha, Operand
     args.nrhs = n;
     args.alpha = alpha;
     args.unit = unit;
+    args.m = m;
     if constexpr (sizeof(T) == 8) {
-        if (m == 24) return quadtrsm::launch_n<T, 24>(args);
+        if (m > 16 && m <= 24) return quadtrsm::launch_n<T, 24>(args);
     }
     else {
-        if (m == 32) return quadtrsm::launch_n<T, 32>(args);
+        if (m > 24 && m <= 32) return quadtrsm::launch_n<T, 32>(args);
     }
     return hipErrorInvalidValue;
 }

@@ -44,6 +44,7 @@ template <class T> struct QuadTrsmArgs {
     int nrhs;
     T alpha;
     bool unit;
+    int m; /* the order, m <= N: rows and columns past it read as the identity */
 };
 
 template <class T, int N, int NRB, bool PREFETCH, bool UNIT>
@@ -65,20 +66,23 @@ quad_trsm_kernel(QuadTrsmArgs<T> p)
     unsigned a_row[K], b_row[K];
     sfor<0, K>([&](auto k) {
         const int i = t + 4 * k;
-        a_row[k] = (aslot == kOOB) ? kOOB : aslot + (unsigned)(i * p.a_si) * VS;
-        b_row[k] = (bslot == kOOB) ? kOOB : bslot + (unsigned)(i * p.b_si) * VS;
+        a_row[k] = (aslot == kOOB || i >= p.m) ? kOOB : aslot + (unsigned)(i * p.a_si) * VS;
+        b_row[k] = (bslot == kOOB || i >= p.m) ? kOOB : bslot + (unsigned)(i * p.b_si) * VS;
     });
 
-    /* op(A) strictly below the diagonal (zeros elsewhere), its diagonal apart */
+    /* op(A) strictly below the diagonal (zeros elsewhere), its diagonal apart;
+     * an order m below N runs as diag(op(A), I) with B's rows past m zero and
+     * never stored: the padded pivots reach only padded rows, so the rows
+     * below m get exactly the order-m operations */
     T a[K][N], dg[K];
     sfor<0, K>([&](auto k) {
         const int i = t + 4 * k;
         sfor<0, 4 * k + 4>([&](auto c) {
-            a[k][c] = bload(T(), ra, (c < 4 * k || c < i) ? a_row[k] : kOOB, c * a_col);
+            a[k][c] = bload(T(), ra, (c < p.m && (c < 4 * k || c < i)) ? a_row[k] : kOOB, c * a_col);
         });
         if constexpr (!UNIT) {
             const unsigned doff = (a_row[k] == kOOB) ? kOOB : a_row[k] + (unsigned)i * a_col;
-            dg[k] = bload(T(), ra, doff, 0);
+            dg[k] = (i >= p.m) ? T(1) : bload(T(), ra, doff, 0);
         }
     });
 

@@ -1,6 +1,7 @@
-"""GPU parity of the trsm team kernel (side L, forward sweeps: uplo L without
-transpose, uplo U transposed; m in {16, 24, 32}) against the oracle, bit for
-bit: the kernel performs the reference's operations on every element in its
+"""GPU parity of the default trsm kernels at m in {16, 20, 24, 28, 32} (the team
+and quad kernels on side L, forward sweeps: uplo L without transpose, uplo U
+transposed; FP64 20 and FP32 28 on the quad kernel's instances of order 24 /
+32 as diag(op(A), I)) against the oracle, bit for bit: the kernel performs the reference's operations on every element in its
 order (alpha first, the fma's over l ascending, then the divide). The backward
 sweeps and side R, which the lane kernel serves, are checked alongside."""
 
@@ -18,7 +19,7 @@ def _gpu():
 
 
 @pytest.mark.parametrize("p", ["d", "s"])
-@pytest.mark.parametrize("m", [16, 24, 32])
+@pytest.mark.parametrize("m", [16, 20, 24, 28, 32])
 def test_trsm_team_bitwise(p, m):
     for lay in ("C", "R"):
         for uplo, tr in (("L", "N"), ("U", "T"), ("U", "C"), ("L", "T"), ("U", "N")):
