@@ -266,6 +266,12 @@ bool qr_wave_supports(int p, int q, int nrhs);
  * nrhs <= 8; two problems per wavefront on 4 x 8 lanes each for q <= 32, one
  * on 8 x 8 lanes above */
 bool qr_grid_supports(int p, int q, int nrhs);
+/* potrf / posv / potrs and the LDL^T routines at orders 33..64, one matrix per
+ * wavefront on an 8 x 8 lane grid (chol_grid.hip) */
+bool chol_grid_supports(int n);
+template <class T>
+hipError_t launch_chol_grid(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
+                            Operand<T> B, int V, int nm, int *info);
 template <class T>
 hipError_t launch_qr_grid(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
                           Operand<T> B, int V, int nm, int *info);

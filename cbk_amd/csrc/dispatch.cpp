@@ -28,6 +28,10 @@ hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand
             if (sym_team_supports(n) && V >= 4)
                 return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
             break;
+        case CBK_FAMILY_GRID:
+            if (chol_grid_supports(n))
+                return launch_chol_grid<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+            break;
         case CBK_FAMILY_LANE: return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
         default: break;
         }
@@ -46,6 +50,10 @@ hipError_t dispatch_sym(bool ldl, bool factor, bool up, int n, int nrhs, Operand
         return launch_quad_sym<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     if (fit && current_policy() == CBK_POLICY_AUTO && sym_team_supports(n) && V >= 4)
         return launch_sym_team<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
+    /* orders 33..64: a matrix per wavefront on the 8 x 8 lane grid (dposv
+     * n = 48: 13.9 ms on the lane kernel) */
+    if (fit && current_policy() != CBK_POLICY_LANE && chol_grid_supports(n))
+        return launch_chol_grid<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
     return launch_chol_lane<T>(ldl, factor, up, n, nrhs, A, B, V, nm, info);
 }
 

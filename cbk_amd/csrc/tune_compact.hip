@@ -104,6 +104,7 @@ std::vector<int> candidates(const Shape &s, int V)
         if (reg_chol_supports(s.p, s.fp64)) f.push_back(CBK_FAMILY_REGISTER);
         if (quad_sym_supports(s.p, s.fp64)) f.push_back(CBK_FAMILY_QUAD);
         if (sym_team_supports(s.p) && V >= 4) f.push_back(CBK_FAMILY_TEAM);
+        if (chol_grid_supports(s.p)) f.push_back(CBK_FAMILY_GRID);
         break;
     case kTuneTrsmFwd:
     case kTuneTrsmBwd: {

@@ -288,7 +288,7 @@ int cbk_set_tuning(const cbk_tuning *t);
 #define CBK_FAMILY_COLUMNS 4  /* one matrix per 16 lanes, columns in lanes (QR) */
 #define CBK_FAMILY_WAVE 5     /* one matrix per wavefront (QR) */
 #define CBK_FAMILY_LANE 6     /* the general one-lane-per-matrix kernel */
-#define CBK_FAMILY_GRID 7     /* QR on lane grids: two problems per wavefront (n <= 32), one above */
+#define CBK_FAMILY_GRID 7     /* lane grids: QR two problems per wavefront (n <= 32), one above; posv / sysvnp 33..64 */
 
 /* Measure and install the fastest family for one shape. routine: "posv"
  * (also potrf / potrs, whose nrhs is 0 / the call's), "sysvnp" (sytrfnp /

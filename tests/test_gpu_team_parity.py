@@ -1,7 +1,8 @@
 """GPU parity of the default kernels (potrf/posv/potrs, sytrfnp/sysvnp/sytrsnp)
-at orders 16 to 32 against the oracle: the quad kernels at 24 and 32, and at
+at orders 16 to 64 against the oracle: the quad kernels at 24 and 32, and at
 20 and 28 their instances of order 24 and 32 running the matrix as
-diag(A, I) (FP64; FP32 order 20 is the register kernel's).
+diag(A, I) (FP64; FP32 order 20 is the register kernel's); the 8 x 8 lane
+grid (one matrix per wavefront) at 33, 40 and 64.
 
 Factors are bit-identical to the oracle's (hence the reference's). Solutions
 share that factor; their backward substitution sums in a different order, so
@@ -21,7 +22,7 @@ import cbkt
 
 pytestmark = pytest.mark.gpu
 
-TEAM_N = (16, 20, 24, 28, 32)
+TEAM_N = (16, 20, 24, 28, 32, 33, 40, 64)
 
 
 def _gpu():
