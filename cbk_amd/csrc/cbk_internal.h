@@ -269,6 +269,11 @@ bool qr_grid_supports(int p, int q, int nrhs);
 /* potrf / posv / potrs and the LDL^T routines at orders 33..64, one matrix per
  * wavefront on an 8 x 8 lane grid (chol_grid.hip) */
 bool chol_grid_supports(int n);
+/* trsm side L, forward sweep, orders 33..64, on the same grid */
+bool trsm_grid_supports(bool left, bool up, bool tran, int m);
+template <class T>
+hipError_t launch_trsm_grid(bool tran, bool unit, int m, int n, T alpha, Operand<T> A, Operand<T> B,
+                            int V, int nm);
 template <class T>
 hipError_t launch_chol_grid(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
                             Operand<T> B, int V, int nm, int *info);

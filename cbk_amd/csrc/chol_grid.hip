@@ -274,9 +274,139 @@ chol_grid_kernel(CholGridArgs<T> p)
     }
 }
 
+/* trsm side L, forward sweep (op(A) lower), orders 33..64: B := alpha
+ * op(A)^{-1} B on 8-column blocks of B, per element the reference's sequence
+ * (src/cbk_trsm_compact.hpp; quad_trsm_impl.h): the multiply by alpha, the
+ * divide on the pivot's own row, then the fma's over l ascending */
+template <class T> struct TrsmGridArgs {
+    const T *a; /* op(A): element (i, c) at (i*a_si + c*a_sj)*V in the group */
+    int a_si, a_sj;
+    size_t a_gs;
+    T *b;
+    int b_si, b_sj;
+    size_t b_gs;
+    int V;
+    long long count;
+    int m, nrhs;
+    T alpha;
+};
+
+template <class T, bool UNIT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
+trsm_grid_kernel(TrsmGridArgs<T> p)
+{
+    constexpr unsigned SZ = sizeof(T);
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 7, c = lane >> 3;
+    const long long kq = (long long)team::xcd_block(blockIdx.x) * 4 + team::wave_in_block();
+    if (kq >= p.count) return; /* whole wavefront */
+    const int m = p.m;
+    const long long g = kq / p.V, v = kq % p.V;
+    const long long groups = p.count / p.V;
+    const __amdgpu_buffer_rsrc_t ra =
+        team::group_rsrc(p.a + g * p.a_gs, team::group_range(groups - g, p.a_gs, SZ));
+    const __amdgpu_buffer_rsrc_t rb =
+        team::group_rsrc(p.b + g * p.b_gs, team::group_range(groups - g, p.b_gs, SZ));
+    const unsigned VS = (unsigned)p.V * SZ;
+    const unsigned ars = (unsigned)p.a_si * VS, acs = (unsigned)p.a_sj * VS;
+    const unsigned brs = (unsigned)p.b_si * VS, bcs = (unsigned)p.b_sj * VS;
+    const unsigned aoff = (unsigned)(v * SZ) + r * ars + c * acs;
+    const unsigned boff = (unsigned)(v * SZ) + r * brs;
+    const int raddr = 4 * r;
+
+    /* op(A) strictly below the diagonal, and the diagonal unless unit */
+    T f[S][S];
+    sfor<0, S>([&](auto a) {
+        sfor<0, a + 1>([&](auto b) {
+            const int i = r + 8 * a, j = c + 8 * b;
+            const bool in = i < m && (j < i || (!UNIT && j == i));
+            f[a][b] = team::bload(T(0), ra, in ? aoff : kOOB, a * 8 * ars + b * 8 * acs);
+        });
+    });
+    for (int q0 = 0; q0 < p.nrhs; q0 += 8) {
+        const int q = q0 + c;
+        const unsigned boq = boff + (unsigned)q * bcs;
+        T bb[S];
+        sfor<0, S>([&](auto a) {
+            const unsigned o = (r + 8 * a < m && q < p.nrhs) ? boq : kOOB;
+            bb[a] = team::bload(T(0), rb, o, a * 8 * brs) * p.alpha;
+        });
+        sfor<0, NMAX>([&](auto lc) {
+            constexpr int l = lc;
+            constexpr int ll = l % 8, lb = l / 8;
+            if (l < m) {
+                if constexpr (!UNIT) {
+                    const T dl = lane_value<ll + 8 * ll>(f[lb][lb]);
+                    bb[lb] = (r == ll) ? bb[lb] / dl : bb[lb];
+                }
+                const T x = rowbc<ll>(bb[lb]);
+                sfor<lb, S>([&](auto a) {
+                    const T lil = bperm(f[a][lb], raddr + 32 * ll); /* op(A)(r + 8a, l) */
+                    if constexpr (a > lb)
+                        bb[a] = fmaT(-lil, x, bb[a]);
+                    else
+                        bb[a] = (r > ll) ? fmaT(-lil, x, bb[a]) : bb[a];
+                });
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
+        const unsigned boq2 = opaque(boq);
+        const int r2 = opaque(r);
+        sfor<0, S>([&](auto a) {
+            const unsigned o = (r2 + 8 * a < m && q < p.nrhs) ? boq2 : kOOB;
+            team::bstore(bb[a], rb, o, a * 8 * brs);
+        });
+    }
+}
+
 } /* namespace cholgrid */
 
 bool chol_grid_supports(int n) { return n > 32 && n <= 64; }
+
+bool trsm_grid_supports(bool left, bool up, bool tran, int m)
+{
+    return left && up == tran && m > 32 && m <= 64;
+}
+
+template <class T>
+hipError_t launch_trsm_grid(bool tran, bool unit, int m, int n, T alpha, Operand<T> A, Operand<T> B,
+                            int V, int nm)
+{
+    cholgrid::TrsmGridArgs<T> args;
+    args.a = A.p;
+    /* op(A): the transposed view when tran (src/cbk_common.hpp:156-177) */
+    int si = A.row ? A.ld : 1, sj = A.row ? 1 : A.ld;
+    if (tran) {
+        const int tmp = si;
+        si = sj;
+        sj = tmp;
+    }
+    args.a_si = si;
+    args.a_sj = sj;
+    args.a_gs = A.gstride;
+    args.b = B.p;
+    args.b_si = B.row ? B.ld : 1;
+    args.b_sj = B.row ? 1 : B.ld;
+    args.b_gs = B.gstride;
+    args.V = V;
+    args.count = (long long)((nm + V - 1) / V) * V;
+    args.m = m;
+    args.nrhs = n;
+    args.alpha = alpha;
+    const unsigned blocks = team::xcd_grid((unsigned)((args.count + 3) / 4));
+    if (unit)
+        hipLaunchKernelGGL((cholgrid::trsm_grid_kernel<T, true>), dim3(blocks), dim3(256), 0,
+                           current_stream(), args);
+    else
+        hipLaunchKernelGGL((cholgrid::trsm_grid_kernel<T, false>), dim3(blocks), dim3(256), 0,
+                           current_stream(), args);
+    return hipGetLastError();
+}
+
+template hipError_t launch_trsm_grid<double>(bool, bool, int, int, double, Operand<double>,
+                                             Operand<double>, int, int);
+template hipError_t launch_trsm_grid<float>(bool, bool, int, int, float, Operand<float>,
+                                            Operand<float>, int, int);
 
 template <class T>
 hipError_t launch_chol_grid(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,

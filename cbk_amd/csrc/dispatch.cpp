@@ -79,6 +79,10 @@ hipError_t dispatch_trsm(bool left, bool up, bool tran, bool unit, int m, int n,
             if (V >= 4 && trsm_team_supports(left, up, tran, m))
                 return launch_trsm_team<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
             break;
+        case CBK_FAMILY_GRID:
+            if (trsm_grid_supports(left, up, tran, m))
+                return launch_trsm_grid<T>(tran, unit, m, n, alpha, A, B, V, nm);
+            break;
         case CBK_FAMILY_LANE:
             return launch_trsm_lane<T>(left, up, tran, unit, m, n, alpha, A, B, V, nm);
         default: break;
@@ -100,6 +104,8 @@ hipError_t dispatch_trsm(bool left, bool up, bool tran, bool unit, int m, int n,
             return launch_quad_trsm<T>(!tran, unit, n, m, alpha, A, Bt, V, nm);
         if (V >= 4 && trsm_team_supports(true, up, !tran, n))
             return launch_trsm_team<T>(up, !tran, unit, n, m, alpha, A, Bt, V, nm);
+        if (trsm_grid_supports(true, up, !tran, n))
+            return launch_trsm_grid<T>(!tran, unit, n, m, alpha, A, Bt, V, nm);
     }
     if (fit && current_policy() != CBK_POLICY_LANE && trsm_reg_supports(left, m, sizeof(T) == 8))
         return launch_trsm_reg<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
@@ -107,6 +113,8 @@ hipError_t dispatch_trsm(bool left, bool up, bool tran, bool unit, int m, int n,
         return launch_quad_trsm<T>(tran, unit, m, n, alpha, A, B, V, nm);
     if (fit && current_policy() != CBK_POLICY_LANE && V >= 4 && trsm_team_supports(left, up, tran, m))
         return launch_trsm_team<T>(up, tran, unit, m, n, alpha, A, B, V, nm);
+    if (fit && current_policy() != CBK_POLICY_LANE && trsm_grid_supports(left, up, tran, m))
+        return launch_trsm_grid<T>(tran, unit, m, n, alpha, A, B, V, nm);
     return launch_trsm_lane<T>(left, up, tran, unit, m, n, alpha, A, B, V, nm);
 }
 

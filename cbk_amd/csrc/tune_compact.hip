@@ -112,6 +112,7 @@ std::vector<int> candidates(const Shape &s, int V)
         if (trsm_reg_supports(true, s.p, s.fp64)) f.push_back(CBK_FAMILY_REGISTER);
         if (quad_trsm_supports(true, up, tran, s.p, s.fp64)) f.push_back(CBK_FAMILY_QUAD);
         if (V >= 4 && trsm_team_supports(true, up, tran, s.p)) f.push_back(CBK_FAMILY_TEAM);
+        if (trsm_grid_supports(true, up, tran, s.p)) f.push_back(CBK_FAMILY_GRID);
         break;
     }
     case kTuneGels:

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("p,n", [("d", 8), ("d", 16), ("d", 20), ("d", 24), ("d", 32), ("s", 20),
-                                 ("s", 24), ("s", 28), ("s", 32), ("d", 5)])
+                                 ("s", 24), ("s", 28), ("s", 32), ("d", 5), ("d", 40), ("s", 64)])
 def test_trsm_side_r_matches_oracle(p, n):
     import gpuutil
     combos = list(itertools.product("CR", "UL", "NT", "UN"))

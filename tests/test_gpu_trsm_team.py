@@ -1,7 +1,8 @@
-"""GPU parity of the default trsm kernels at m in {16, 20, 24, 28, 32} (the team
-and quad kernels on side L, forward sweeps: uplo L without transpose, uplo U
-transposed; FP64 20 and FP32 28 on the quad kernel's instances of order 24 /
-32 as diag(op(A), I)) against the oracle, bit for bit: the kernel performs the reference's operations on every element in its
+"""GPU parity of the default trsm kernels at m in {16, 20, 24, 28, 32, 40, 64}
+(the team and quad kernels on side L, forward sweeps: uplo L without
+transpose, uplo U transposed; FP64 20 and FP32 28 on the quad kernel's
+instances of order 24 / 32 as diag(op(A), I); 40 and 64 on the 8 x 8 lane
+grid) against the oracle, bit for bit: the kernel performs the reference's operations on every element in its
 order (alpha first, the fma's over l ascending, then the divide). The backward
 sweeps and side R, which the lane kernel serves, are checked alongside."""
 
@@ -19,7 +20,7 @@ def _gpu():
 
 
 @pytest.mark.parametrize("p", ["d", "s"])
-@pytest.mark.parametrize("m", [16, 20, 24, 28, 32])
+@pytest.mark.parametrize("m", [16, 20, 24, 28, 32, 40, 64])
 def test_trsm_team_bitwise(p, m):
     for lay in ("C", "R"):
         for uplo, tr in (("L", "N"), ("U", "T"), ("U", "C"), ("L", "T"), ("U", "N")):
