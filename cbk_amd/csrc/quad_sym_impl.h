@@ -70,7 +70,9 @@ template <class T, int N> constexpr bool prefetch_b() { return !(sizeof(T) == 8 
 /* MASKED: the instance that the masked passes after the mixed-precision
  * kernel launch (QuadSymArgs::only set), a kernel of its own so that traces
  * keep the two apart; either instance honours only */
-template <class T, int N, int NRB, bool LDL, bool MASKED = false>
+/* PAD: the instance for orders below N (run as diag(A, I)); the one for N
+ * itself folds the order in, so its loads and stores carry no order masks */
+template <class T, int N, int NRB, bool LDL, bool MASKED = false, bool PAD = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_simd<T, N>(),
                                                                      waves_per_simd<T, N>()))) void
 quad_sym_kernel(QuadSymArgs<T> p)
@@ -94,7 +96,7 @@ quad_sym_kernel(QuadSymArgs<T> p)
     const unsigned aslot = pl.slot(p.a_gs, p.V, SZ);
     const unsigned a_col = (unsigned)p.a_sj * VS;
     unsigned a_row[K]; /* byte offset of (row t + 4k, column 0) */
-    const int n = p.n;
+    const int n = PAD ? p.n : N;
     sfor<0, K>([&](auto k) {
         a_row[k] = (aslot == kOOB || t + 4 * k >= n) ? kOOB
                                                      : aslot + (unsigned)((t + 4 * k) * p.a_si) * VS;
@@ -198,7 +200,8 @@ quad_sym_kernel(QuadSymArgs<T> p)
     }
 }
 
-template <class T, int N, int NRB, bool LDL, bool MASKED> hipError_t launch_nrb_m(const QuadSymArgs<T> &args)
+template <class T, int N, int NRB, bool LDL, bool MASKED, bool PAD>
+hipError_t launch_nrb_mp(const QuadSymArgs<T> &args)
 {
     const long waves = (args.nmat + kMatsPerWave - 1) / kMatsPerWave;
     const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
@@ -206,7 +209,7 @@ template <class T, int N, int NRB, bool LDL, bool MASKED> hipError_t launch_nrb_
         QuadSymArgs<T> a1 = args;
         a1.q0 = 0;
         a1.skip = nullptr;
-        hipLaunchKernelGGL((quad_sym_kernel<T, N, NRB, LDL, MASKED>), grid, block, 0, current_stream(), a1);
+        hipLaunchKernelGGL((quad_sym_kernel<T, N, NRB, LDL, MASKED, PAD>), grid, block, 0, current_stream(), a1);
         return hipGetLastError();
     }
     /* one launch per block of NRB right-hand sides; the factor from the first */
@@ -216,11 +219,18 @@ template <class T, int N, int NRB, bool LDL, bool MASKED> hipError_t launch_nrb_
         a1.factor = args.factor && q0 == 0;
         a1.skip = (q0 > 0 && args.factor) ? args.info : nullptr;
         if (q0 > 0) a1.info = nullptr;
-        hipLaunchKernelGGL((quad_sym_kernel<T, N, NRB, LDL, MASKED>), grid, block, 0, current_stream(), a1);
+        hipLaunchKernelGGL((quad_sym_kernel<T, N, NRB, LDL, MASKED, PAD>), grid, block, 0, current_stream(), a1);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+template <class T, int N, int NRB, bool LDL, bool MASKED>
+hipError_t launch_nrb_m(const QuadSymArgs<T> &args)
+{
+    if (args.n != N) return launch_nrb_mp<T, N, NRB, LDL, MASKED, true>(args);
+    return launch_nrb_mp<T, N, NRB, LDL, MASKED, false>(args);
 }
 
 template <class T, int N, int NRB, bool LDL> hipError_t launch_nrb(const QuadSymArgs<T> &args)
