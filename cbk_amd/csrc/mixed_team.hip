@@ -12,7 +12,9 @@ extern template hipError_t launch_n<24>(bool, const MixedArgs &);
 extern template hipError_t launch_n<32>(bool, const MixedArgs &);
 } /* namespace mixedteam */
 
-bool mixed_team_supports(int n) { return n == 16 || n == 24 || n == 32; }
+/* orders 1..32: the kernels of orders 16, 24 and 32, any other order padded
+ * with the identity up to the next of them */
+bool mixed_team_supports(int n) { return n >= 1 && n <= 32; }
 
 hipError_t launch_mixed_posv_team(bool ldl, bool up, int n, int nrhs, Operand<double> A,
                                   Operand<double> B, Operand<double> X, int V, int nm,
@@ -40,16 +42,15 @@ hipError_t launch_mixed_posv_team(bool ldl, bool up, int n, int nrhs, Operand<do
     p.V = V;
     p.nmat = (long)((nm + V - 1) / V) * V;
     p.nm = nm;
+    p.n = n;
     p.nrhs = nrhs;
     p.itermax = itermax;
     p.iter = iter;
     p.info = info;
-    switch (n) {
-    case 16: return mixedteam::launch_n<16>(ldl, p);
-    case 24: return mixedteam::launch_n<24>(ldl, p);
-    case 32: return mixedteam::launch_n<32>(ldl, p);
-    default: return hipErrorInvalidValue;
-    }
+    if (!mixed_team_supports(n)) return hipErrorInvalidValue;
+    if (n <= 16) return mixedteam::launch_n<16>(ldl, p);
+    if (n <= 24) return mixedteam::launch_n<24>(ldl, p);
+    return mixedteam::launch_n<32>(ldl, p);
 }
 
 } /* namespace cbk */

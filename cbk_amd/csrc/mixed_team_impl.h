@@ -46,6 +46,7 @@ struct MixedArgs {
     int V;
     long nmat;
     int nm;
+    int n; /* the order: N, or below N when padded */
     int nrhs;
     int itermax;
     int *iter;
@@ -90,12 +91,19 @@ __device__ __forceinline__ void team_max4(double (&x)[4], int t)
     x[3] = bcast<3>(k);
 }
 
-template <int N, int NRB, bool LDL>
+/* PAD: an order n < N solved as diag(A, I) of order N -- rows and columns
+ * from n on load zero and a unit diagonal, nothing is stored there, and the
+ * norms, the stopping test and its sqrt(n) use the n real rows. The factor's
+ * entries of A and the residual sums of its rows are those of order n (the
+ * padded columns follow the real ones and add exact zeros); the unpadded
+ * instances keep N as a constant. */
+template <int N, int NRB, bool LDL, bool PAD = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N <= 16 ? 4 : 2, N <= 16 ? 4 : 2))) void
 mixed_posv_kernel(MixedArgs p)
 {
     static_assert(NRB == 4, "the convergence test reduces four columns at once");
     constexpr int R = rows_per_lane<N>();
+    const int n = PAD ? p.n : N;
     /* a team's region: the FP32 column copy during refinement, the FP64
      * packed rows of chol_prepare on the fallback */
     constexpr int TW = chol_lds<N>() > (cols_lds<N>() + 1) / 2 ? chol_lds<N>() : (cols_lds<N>() + 1) / 2;
@@ -119,7 +127,7 @@ mixed_posv_kernel(MixedArgs p)
     unsigned a_row[R], a_cbase[R], b_row[R], x_row[R];
     sfor<0, R>([&](auto r) {
         const int i = t + TS * r;
-        const bool in = i < N;
+        const bool in = i < n;
         a_row[r] = in ? ((unsigned)(i * p.a_si) * V + v) * SZ : kOOB;   /* (i, c): + c*a_col */
         a_cbase[r] = in ? ((unsigned)(i * p.a_sj) * V + v) * SZ : kOOB; /* (c, i): + c*a_rowb */
         b_row[r] = in ? ((unsigned)(i * p.b_si) * V + v) * SZ : kOOB;
@@ -132,10 +140,12 @@ mixed_posv_kernel(MixedArgs p)
     sfor<0, R>([&](auto r) {
         const int i = t + TS * r;
         sfor<0, N>([&](auto c) {
-            const unsigned off = (a_row[r] == kOOB) ? kOOB
-                                 : (c <= i)         ? a_row[r] + c * a_col
-                                                    : a_cbase[r] + c * a_rowb;
+            const unsigned off = (a_row[r] == kOOB || (PAD && c >= n)) ? kOOB
+                                 : (c <= i)                           ? a_row[r] + c * a_col
+                                                                      : a_cbase[r] + c * a_rowb;
             a64[r][c] = bload(0.0, ra, off, 0);
+            if constexpr (PAD && c >= TS * r && c < TS * (r + 1))
+                a64[r][c] = (c >= n && c == i) ? 1.0 : a64[r][c];
         });
     });
     /* ||A||_inf (dlansy 'I') and the FP32 overflow test of A (dlat2s) */
@@ -150,10 +160,10 @@ mixed_posv_kernel(MixedArgs p)
     });
     double anrm = 0.0;
     sfor<0, R>([&](auto r) {
-        if (t + TS * r < N) anrm = fmax_nan(anrm, rowsum[r]);
+        if (t + TS * r < n) anrm = fmax_nan(anrm, rowsum[r]);
     });
     anrm = team_max(anrm);
-    const double cte = anrm * 0x1p-53 * __builtin_sqrt((double)N);
+    const double cte = anrm * 0x1p-53 * __builtin_sqrt((double)n);
     /* any lane of the team overflowing sends the whole matrix to FP64 */
     const bool a_over = team_any(over);
 
@@ -173,7 +183,7 @@ mixed_posv_kernel(MixedArgs p)
             finfo = chol_factor<float, N, true>(a32, dg32, t);
             bool bad = false;
             sfor<0, R>([&](auto r) {
-                if (t + TS * r < N) bad |= !__builtin_isfinite(dg32[r]);
+                if (t + TS * r < n) bad |= !__builtin_isfinite(dg32[r]);
             });
             if (team_any(bad)) finfo = 1;
         }
@@ -239,14 +249,14 @@ mixed_posv_kernel(MixedArgs p)
             sfor<0, NRB>([&](auto q) {
                 xm[q] = 0.0;
                 sfor<0, R>([&](auto r) {
-                    if (t + TS * r < N) xm[q] = fmax_nan(xm[q], __builtin_fabs(x64[r][q]));
+                    if (t + TS * r < n) xm[q] = fmax_nan(xm[q], __builtin_fabs(x64[r][q]));
                 });
             });
             team_max4(xm, t);
             bool fail = false;
             sfor<0, NRB>([&](auto q) {
                 sfor<0, R>([&](auto r) {
-                    if (q0 + q < p.nrhs && t + TS * r < N)
+                    if (q0 + q < p.nrhs && t + TS * r < n)
                         fail = fail || !(__builtin_fabs(res[r][q]) <= xm[q] * cte);
                 });
             });
@@ -326,15 +336,20 @@ mixed_posv_kernel(MixedArgs p)
     }
 }
 
-template <int N> hipError_t launch_n(bool ldl, const MixedArgs &args)
+template <int N, bool LDL, bool PAD> void launch_np(const MixedArgs &args)
 {
     const unsigned blocks = (unsigned)((args.nmat + kTeamsPerBlock - 1) / kTeamsPerBlock);
+    hipLaunchKernelGGL((mixed_posv_kernel<N, 4, LDL, PAD>), dim3(xcd_grid(blocks)), dim3(64), 0,
+                       current_stream(), args);
+}
+
+template <int N> hipError_t launch_n(bool ldl, const MixedArgs &args)
+{
+    const bool pad = args.n != N;
     if (ldl)
-        hipLaunchKernelGGL((mixed_posv_kernel<N, 4, true>), dim3(xcd_grid(blocks)), dim3(64), 0,
-                           current_stream(), args);
+        pad ? launch_np<N, true, true>(args) : launch_np<N, true, false>(args);
     else
-        hipLaunchKernelGGL((mixed_posv_kernel<N, 4, false>), dim3(xcd_grid(blocks)), dim3(64), 0,
-                           current_stream(), args);
+        pad ? launch_np<N, false, true>(args) : launch_np<N, false, false>(args);
     return hipGetLastError();
 }
 

@@ -17,7 +17,8 @@ Checks, per matrix:
     FLT_MAX; -3: a diagonal that is 0 in FP32, for sysvnp also a D whose FP32
     reciprocal overflows), equal to the oracle's, X then bit-identical to the
     oracle's FP64 solve.
-Both the fused team kernel (n in {16,24,32}, any nrhs in blocks of 4) and the
+Both the fused kernels (n in 1..32, any nrhs in blocks of 4; orders other
+than 16, 24 and 32 padded with the identity up to the next of them) and the
 workspace lane kernel (other shapes, or the lane policy) are exercised.
 """
 
@@ -82,8 +83,13 @@ def cond1(A):
     (24, 3, 8, "U", "C", False),
     (16, 1, 4, "L", "R", False),
     (32, 4, 64, "L", "C", True),    # the same shape through the workspace lane kernel
-    (20, 5, 8, "L", "C", False),    # an order only the lane kernel serves
-    (7, 2, 2, "U", "R", False),
+    (20, 5, 8, "L", "C", False),    # orders between 16, 24 and 32: padded with the identity
+    (28, 4, 64, "L", "C", False),
+    (12, 3, 8, "U", "R", False),
+    (20, 5, 8, "L", "C", True),     # the same order through the lane kernel
+    (7, 2, 2, "U", "R", False),     # padded up to 16
+    (7, 2, 2, "U", "R", True),
+    (1, 3, 4, "L", "C", False),
     (32, 4, 32, "L", "C", False),   # V = 32: the fused kernel's matrices span two per group line
     (24, 2, 32, "U", "R", False),
     (32, 9, 64, "L", "C", False),   # nrhs > 4: the fused kernel in blocks of 4 columns
@@ -120,7 +126,8 @@ def test_dsposv_refines_to_fp64_accuracy(n, nrhs, V, uplo, layout, lane):
     assert np.all(ratio <= 1.0), ratio.max()
 
 
-@pytest.mark.parametrize("n,V,lane", [(32, 64, False), (32, 64, True), (24, 8, False)])
+@pytest.mark.parametrize("n,V,lane", [(32, 64, False), (32, 64, True), (24, 8, False),
+                                      (28, 8, False), (14, 8, False)])
 def test_dsposv_fallback_and_info(n, V, lane):
     import cbk_amd
     r = cbkt.rng(7 + n)
@@ -172,7 +179,9 @@ def sym_indefinite_logspace(r, nm, n, logk):
     (32, 4, 64, "L", "C", False),
     (16, 1, 4, "U", "R", False),
     (24, 24, 64, "L", "C", True),   # the same through the workspace lane kernel
-    (20, 5, 8, "L", "C", False),    # an order only the lane kernel serves
+    (20, 5, 8, "L", "C", False),    # padded with the identity up to 24
+    (30, 4, 16, "U", "R", False),   # up to 32
+    (20, 5, 8, "L", "C", True),     # the lane kernel
     (24, 6, 32, "U", "C", False),
 ])
 def test_dssysvnp_refines_to_fp64_accuracy(n, nrhs, V, uplo, layout, lane):
@@ -204,7 +213,8 @@ def test_dssysvnp_refines_to_fp64_accuracy(n, nrhs, V, uplo, layout, lane):
     assert np.all(ratio <= 1.0), ratio.max()
 
 
-@pytest.mark.parametrize("n,V,lane", [(24, 64, False), (24, 64, True), (32, 8, False)])
+@pytest.mark.parametrize("n,V,lane", [(24, 64, False), (24, 64, True), (32, 8, False),
+                                      (21, 8, False), (27, 8, False)])
 def test_dssysvnp_fallback_and_info(n, V, lane):
     import cbk_amd
     r = cbkt.rng(17 + n)
@@ -266,7 +276,8 @@ def iter_code_cases(n, ldl):
 
 @pytest.mark.parametrize("name", ["dsposv_compact", "dssysvnp_compact"])
 @pytest.mark.parametrize("n,V,lane", [(16, 8, False), (32, 4, False), (6, 8, False),
-                                      (16, 8, True)])
+                                      (6, 8, True), (16, 8, True), (11, 8, False),
+                                      (29, 4, False)])
 def test_mixed_iter_codes_match_oracle(name, n, V, lane):
     import cbk_amd
     ldl = name == "dssysvnp_compact"
