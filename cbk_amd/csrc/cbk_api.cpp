@@ -97,6 +97,8 @@ int ormqr(char layout, char side, char trans, int m, int n, int k, const T *ap, 
         return status(launch_ormqr_reg<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
     if (fit && current_policy() != CBK_POLICY_LANE && V >= 4 && ormqr_team_supports(nq))
         return status(launch_ormqr_team<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
+    if (fit && current_policy() != CBK_POLICY_LANE && apply_grid_supports(nq))
+        return status(launch_ormqr_grid<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
     return status(launch_ormqr_lane<T>(left, opt(trans, 'T'), m, n, k, A, taup, C, V, nm));
 }
 
@@ -114,9 +116,11 @@ int orgqr(char layout, int m, int n, int k, T *ap, int ldap, const T *taup, int 
     if (nm < 0) return -9;
     if (m == 0 || n == 0 || nm == 0) return 0; /* k == 0 is not empty */
     const Operand<T> A = operand(ap, ldap, row, m, n, V);
-    if (offsets_fit(A, m, n, V) && current_policy() != CBK_POLICY_LANE &&
-        qr_reg_supports(m, n, sizeof(T) == 8))
+    const bool fit = offsets_fit(A, m, n, V) && current_policy() != CBK_POLICY_LANE;
+    if (fit && qr_reg_supports(m, n, sizeof(T) == 8))
         return status(launch_orgqr_reg<T>(m, n, k, A, taup, V, nm));
+    if (fit && apply_grid_supports(m))
+        return status(launch_orgqr_grid<T>(m, n, k, A, taup, V, nm));
     return status(launch_orgqr_lane<T>(m, n, k, A, taup, V, nm));
 }
 

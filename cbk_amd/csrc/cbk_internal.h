@@ -140,6 +140,15 @@ template <class T>
 hipError_t launch_ormqr_team(bool left, bool tran, int m, int n, int k, Operand<T> A,
                              const T *tau, Operand<T> C, int V, int nm);
 
+/* ormqr / orgqr on the 8 x 8 lane grid, one problem per wavefront
+ * (qr_apply64.hip): reflector lengths (orgqr: m) up to 64 */
+bool apply_grid_supports(int nq);
+template <class T>
+hipError_t launch_ormqr_grid(bool left, bool tran, int m, int n, int k, Operand<T> A,
+                             const T *tau, Operand<T> C, int V, int nm);
+template <class T>
+hipError_t launch_orgqr_grid(int m, int n, int k, Operand<T> A, const T *tau, int V, int nm);
+
 /* orders served by the fused mixed posv / sysvnp team kernel (any nrhs, in
  * blocks of 4 right-hand sides) */
 bool mixed_team_supports(int n);
