@@ -60,6 +60,31 @@ __device__ __forceinline__ double shfl_xor_d(double v, int mask)
 
 /* ---- prep ------------------------------------------------------------------ */
 
+/* dst(i, j) := src(i, j) for the rows x cols block of a column-major problem
+ * (leading dimension ld): the block's elements dealt over the wavefront's
+ * lanes in column order, four loads in flight a lane, so that a problem costs
+ * a few memory round trips rather than one per column */
+template <class S, class D>
+__device__ __forceinline__ void copy_block(D *dst, const S *src, int rows, int cols, int ld, int lane)
+{
+    constexpr int U = 4;
+    const int total = rows * cols;
+    for (int e0 = 0; e0 < total; e0 += 64 * U) {
+        long long off[U];
+        S v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + lane + 64 * u;
+            const int c = e < total ? e / rows : 0;
+            off[u] = e < total ? (long long)c * ld + (e - c * rows) : -1;
+            v[u] = off[u] >= 0 ? src[off[u]] : S(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (off[u] >= 0) dst[off[u]] = (D)v[u];
+    }
+}
+
 /* one wavefront per problem: convert (mode 0, MixedProblem list) or copy B's
  * max(m, n) x nrhs block into X (mode 1, BatchProblem list) */
 __global__ __launch_bounds__(256) void mixed_prep_kernel(MixedPrepArgs p)
@@ -69,19 +94,13 @@ __global__ __launch_bounds__(256) void mixed_prep_kernel(MixedPrepArgs p)
     if (k >= p.count) return;
     if (p.conv) {
         const MixedProblem d = p.conv[k];
-        for (int c = 0; c < d.n; ++c)
-            for (int i = lane; i < d.m; i += 64)
-                p.a32[d.a_off + (long long)c * d.lda + i] = (float)p.a[d.a_off + (long long)c * d.lda + i];
-        for (int q = 0; q < d.nrhs; ++q)
-            for (int i = lane; i < d.m; i += 64)
-                p.b32[d.b_off + (long long)q * d.ldb + i] = (float)p.b[d.b_off + (long long)q * d.ldb + i];
+        copy_block(p.a32 + d.a_off, p.a + d.a_off, d.m, d.n, d.lda, lane);
+        copy_block(p.b32 + d.b_off, p.b + d.b_off, d.m, d.nrhs, d.ldb, lane);
     }
     else {
         const BatchProblem d = p.copy[k];
         const int rows = d.m > d.n ? d.m : d.n;
-        for (int q = 0; q < d.nrhs; ++q)
-            for (int i = lane; i < rows; i += 64)
-                p.x[d.b_off + (long long)q * d.ldb + i] = p.b[d.b_off + (long long)q * d.ldb + i];
+        copy_block(p.x + d.b_off, p.b + d.b_off, rows, d.nrhs, d.ldb, lane);
         if (lane == 0) p.iter[d.idx] = -1;
     }
 }
