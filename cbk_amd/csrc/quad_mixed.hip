@@ -10,7 +10,9 @@ namespace quadmixed {
 extern template hipError_t launch_n<24, nrb_of(24)>(bool, const QuadMixedArgs &);
 } /* namespace quadmixed */
 
-bool quad_mixed_supports(int n) { return n == 24; }
+/* orders 17..24: the kernel of order 24, an order below it padded with the
+ * identity */
+bool quad_mixed_supports(int n) { return n > 16 && n <= 24; }
 
 hipError_t launch_mixed_posv_quad(bool ldl, bool up, int n, int nrhs, Operand<double> A,
                                   Operand<double> B, Operand<double> X, int V, int nm,
@@ -39,15 +41,13 @@ hipError_t launch_mixed_posv_quad(bool ldl, bool up, int n, int nrhs, Operand<do
     p.V = V;
     p.nmat = (long)((nm + V - 1) / V) * V;
     p.nm = nm;
+    p.n = n;
     p.nrhs = nrhs;
     p.itermax = itermax;
     p.iter = iter;
     p.info = info;
-    hipError_t e;
-    switch (n) {
-    case 24: e = quadmixed::launch_n<24, quadmixed::nrb_of(24)>(ldl, p); break;
-    default: return hipErrorInvalidValue;
-    }
+    if (!quad_mixed_supports(n)) return hipErrorInvalidValue;
+    hipError_t e = quadmixed::launch_n<24, quadmixed::nrb_of(24)>(ldl, p);
     if (e != hipSuccess) return e;
     /* the FP64 posv / sysvnp on the matrices marked iter < 0, in place on A and
      * on X (which holds B for them): A receives the factor, info its code */

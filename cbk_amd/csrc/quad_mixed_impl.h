@@ -58,6 +58,7 @@ struct QuadMixedArgs {
     int V;
     long nmat;
     int nm;
+    int n; /* the order: N, or below N when padded */
     int nrhs;
     int itermax;
     int *iter;
@@ -253,7 +254,11 @@ __device__ __forceinline__ void solve32(float (&b)[N / 4][NRB], const float (&di
     }
 }
 
-template <int N, int NRB, bool LDL>
+/* PAD: an order n < N solved as diag(A, I) of order N -- rows and columns
+ * from n on read as zero (the FP32 factor's diagonal there as 1), nothing is
+ * stored there, and the stopping test's rows and sqrt(n) are the n real ones;
+ * the padded rows keep b = x = r = 0 */
+template <int N, int NRB, bool LDL, bool PAD = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void
 quad_mixed_kernel(QuadMixedArgs p)
 {
@@ -266,6 +271,7 @@ quad_mixed_kernel(QuadMixedArgs p)
     QuadPlace pl;
     if (!quad_place(p.nmat, p.V, pl)) return; /* whole wavefront */
     const int t = pl.t;
+    const int n = PAD ? p.n : N;
     const unsigned VS = (unsigned)p.V * SZ;
     const __amdgpu_buffer_rsrc_t ra =
         group_rsrc(p.a + pl.w0 * p.a_gs, group_range(pl.groups_left, p.a_gs, SZ));
@@ -280,8 +286,9 @@ quad_mixed_kernel(QuadMixedArgs p)
     const unsigned b_col = (unsigned)p.b_sj * VS, x_col = (unsigned)p.x_sj * VS;
     unsigned b_row[K], x_row[K];
     sfor<0, K>([&](auto k) {
-        b_row[k] = (bslot == kOOB) ? kOOB : bslot + (unsigned)((t + 4 * k) * p.b_si) * VS;
-        x_row[k] = (xslot == kOOB) ? kOOB : xslot + (unsigned)((t + 4 * k) * p.x_si) * VS;
+        const bool in = !PAD || t + 4 * k < n;
+        b_row[k] = (bslot == kOOB || !in) ? kOOB : bslot + (unsigned)((t + 4 * k) * p.b_si) * VS;
+        x_row[k] = (xslot == kOOB || !in) ? kOOB : xslot + (unsigned)((t + 4 * k) * p.x_si) * VS;
     });
 
     const bool live = aslot != kOOB;
@@ -295,11 +302,15 @@ quad_mixed_kernel(QuadMixedArgs p)
          * on and right of the diagonal */
         float a32[K][N];
         unsigned arow[K];
-        sfor<0, K>([&](auto k) { arow[k] = live ? aslot + (unsigned)(t + 4 * k) * a_rowb : kOOB; });
+        sfor<0, K>([&](auto k) {
+            arow[k] = (live && (!PAD || t + 4 * k < n)) ? aslot + (unsigned)(t + 4 * k) * a_rowb : kOOB;
+        });
         sfor<0, K>([&](auto k) {
             sfor<0, 4 * k + 4>([&](auto c) {
                 a32[k][c] = (float)bload(0.0, ra, (c < 4 * k || c <= t + 4 * k) ? arow[k] : kOOB,
                                          (unsigned)c * a_col);
+                if constexpr (PAD && c >= 4 * k)
+                    a32[k][c] = (c >= n && c == t + 4 * k) ? 1.0f : a32[k][c];
             });
         });
         if (factor32<N, LDL, quad_mfma<float, N>()>(a32, dinv, t)) {
@@ -349,6 +360,8 @@ quad_mixed_kernel(QuadMixedArgs p)
                         off = own[j] ? o1 : o2;
                     }
                     off = live ? off : kOOB;
+                    if constexpr (PAD) /* row t + 4k and column 4m + (t + j) mod 4 (its mirror likewise) */
+                        off = (t + 4 * k < n && 4 * m + ((t + j) & 3) < n) ? off : kOOB;
                     a64[k][4 * m + j] = bload(0.0, ra, off, (unsigned)(4 * m) * a_col);
                 });
             });
@@ -366,7 +379,7 @@ quad_mixed_kernel(QuadMixedArgs p)
         sfor<0, 4 * k + 4>([&](auto c) { over |= __builtin_fabs(a64[k][c]) > (double)FMAX; });
     });
     anrm = quad_max(anrm);
-    const double cte = anrm * 0x1p-53 * __builtin_sqrt((double)N);
+    const double cte = anrm * 0x1p-53 * __builtin_sqrt((double)n);
     if (quad_any(over)) {
         iter = -2;
         failed = true;
@@ -459,14 +472,20 @@ quad_mixed_kernel(QuadMixedArgs p)
     }
 }
 
-template <int N, int NRB> hipError_t launch_n(bool ldl, const QuadMixedArgs &args)
+template <int N, int NRB, bool LDL, bool PAD> void launch_np(const QuadMixedArgs &args)
 {
     const long waves = (args.nmat + kMatsPerWave - 1) / kMatsPerWave;
     const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+    hipLaunchKernelGGL((quad_mixed_kernel<N, NRB, LDL, PAD>), grid, block, 0, current_stream(), args);
+}
+
+template <int N, int NRB> hipError_t launch_n(bool ldl, const QuadMixedArgs &args)
+{
+    const bool pad = args.n != N;
     if (ldl)
-        hipLaunchKernelGGL((quad_mixed_kernel<N, NRB, true>), grid, block, 0, current_stream(), args);
+        pad ? launch_np<N, NRB, true, true>(args) : launch_np<N, NRB, true, false>(args);
     else
-        hipLaunchKernelGGL((quad_mixed_kernel<N, NRB, false>), grid, block, 0, current_stream(), args);
+        pad ? launch_np<N, NRB, false, true>(args) : launch_np<N, NRB, false, false>(args);
     return hipGetLastError();
 }
 

@@ -17,8 +17,9 @@ Checks, per matrix:
     FLT_MAX; -3: a diagonal that is 0 in FP32, for sysvnp also a D whose FP32
     reciprocal overflows), equal to the oracle's, X then bit-identical to the
     oracle's FP64 solve.
-Both the fused kernels (n in 1..32, any nrhs in blocks of 4; orders other
-than 16, 24 and 32 padded with the identity up to the next of them) and the
+Both the fused kernels (n in 1..32, any nrhs in blocks; the quad kernel at
+17..24, the team kernel at the other orders, an order padded with the identity
+up to its kernel's: 24, or 16 / 24 / 32) and the
 workspace lane kernel (other shapes, or the lane policy) are exercised.
 """
 
@@ -84,6 +85,8 @@ def cond1(A):
     (16, 1, 4, "L", "R", False),
     (32, 4, 64, "L", "C", True),    # the same shape through the workspace lane kernel
     (20, 5, 8, "L", "C", False),    # orders between 16, 24 and 32: padded with the identity
+    (17, 3, 64, "U", "R", False),   # (17..23 on the quad kernel, the others on the team kernel)
+    (23, 24, 16, "L", "C", False),
     (28, 4, 64, "L", "C", False),
     (12, 3, 8, "U", "R", False),
     (20, 5, 8, "L", "C", True),     # the same order through the lane kernel
@@ -127,7 +130,7 @@ def test_dsposv_refines_to_fp64_accuracy(n, nrhs, V, uplo, layout, lane):
 
 
 @pytest.mark.parametrize("n,V,lane", [(32, 64, False), (32, 64, True), (24, 8, False),
-                                      (28, 8, False), (14, 8, False)])
+                                      (28, 8, False), (14, 8, False), (20, 8, False)])
 def test_dsposv_fallback_and_info(n, V, lane):
     import cbk_amd
     r = cbkt.rng(7 + n)
@@ -277,7 +280,7 @@ def iter_code_cases(n, ldl):
 @pytest.mark.parametrize("name", ["dsposv_compact", "dssysvnp_compact"])
 @pytest.mark.parametrize("n,V,lane", [(16, 8, False), (32, 4, False), (6, 8, False),
                                       (6, 8, True), (16, 8, True), (11, 8, False),
-                                      (29, 4, False)])
+                                      (19, 8, False), (29, 4, False)])
 def test_mixed_iter_codes_match_oracle(name, n, V, lane):
     import cbk_amd
     ldl = name == "dssysvnp_compact"
