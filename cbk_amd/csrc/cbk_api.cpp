@@ -270,7 +270,7 @@ constexpr int kItermax = 30; /* LAPACK dsposv's ITERMAX */
 bool mixed_uses_team(bool row, int n, int nrhs, int ldap, int ldbp, int ldxp, int V)
 {
     return current_policy() == CBK_POLICY_AUTO && V >= 4 &&
-           (mixed_team_supports(n) || quad_mixed_supports(n)) &&
+           (mixed_team_supports(n) || quad_mixed_supports(n) || mixed_rows_supports(n)) &&
            offsets_fit_bytes(gstride(row, ldap, n, n, V), ldap, row, n, n, V, 8) &&
            offsets_fit_bytes(gstride(row, ldbp, n, nrhs, V), ldbp, row, n, nrhs, V, 8) &&
            offsets_fit_bytes(gstride(row, ldxp, n, nrhs, V), ldxp, row, n, nrhs, V, 8);
@@ -317,6 +317,9 @@ int mixed_sv(bool ldl, char layout, char uplo, int n, int nrhs, double *ap, int 
         if (quad_mixed_supports(n))
             return status(
                 launch_mixed_posv_quad(ldl, up, n, nrhs, A, B, X, V, nm, kItermax, iter, info));
+        if (mixed_rows_supports(n))
+            return status(
+                launch_mixed_posv_rows(ldl, up, n, nrhs, A, B, X, V, nm, kItermax, iter, info));
         return status(
             launch_mixed_posv_team(ldl, up, n, nrhs, A, B, X, V, nm, kItermax, iter, info));
     }

@@ -283,9 +283,17 @@ bool trsm_grid_supports(bool left, bool up, bool tran, int m);
 template <class T>
 hipError_t launch_trsm_grid(bool tran, bool unit, int m, int n, T alpha, Operand<T> A, Operand<T> B,
                             int V, int nm);
+/* mask: if set, only the matrices k < nm with mask[k] < 0 (the FP64 pass
+ * after the mixed kernel of mixed_rows.hip) */
 template <class T>
 hipError_t launch_chol_grid(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
-                            Operand<T> B, int V, int nm, int *info);
+                            Operand<T> B, int V, int nm, int *info, const int *mask = nullptr);
+/* dsposv / dssysvnp at orders 33..64, one matrix per wavefront, lane i holding
+ * row i (mixed_rows.hip); the FP64 pass on the matrices it marks follows */
+bool mixed_rows_supports(int n);
+hipError_t launch_mixed_posv_rows(bool ldl, bool up, int n, int nrhs, Operand<double> A,
+                                  Operand<double> B, Operand<double> X, int V, int nm,
+                                  int itermax, int *iter, int *info);
 template <class T>
 hipError_t launch_qr_grid(int p, int q, int nrhs, bool transposed, Operand<T> A, T *tau,
                           Operand<T> B, int V, int nm, int *info);

@@ -53,6 +53,7 @@ template <class T> struct CholGridArgs {
     int nm, n, nrhs;
     int *info;
     bool factor; /* false: potrs / sytrsnp on a given factor */
+    const int *mask; /* if set: only the matrices k < nm with mask[k] < 0 */
 };
 
 /* x of the lane whose ds_bpermute byte address is addr */
@@ -113,6 +114,7 @@ chol_grid_kernel(CholGridArgs<T> p)
      * share an XCD (a 128-byte run of each element at V >= 16 in FP64) */
     const long long kq = (long long)team::xcd_block(blockIdx.x) * 4 + team::wave_in_block();
     if (kq >= p.count) return; /* whole wavefront */
+    if (p.mask && (kq >= p.nm || p.mask[kq] >= 0)) return;
     const int n = p.n;
     const long long g = kq / p.V, v = kq % p.V;
     const long long groups = p.count / p.V;
@@ -410,7 +412,7 @@ template hipError_t launch_trsm_grid<float>(bool, bool, int, int, float, Operand
 
 template <class T>
 hipError_t launch_chol_grid(bool ldl, bool factor, bool up, int n, int nrhs, Operand<T> A,
-                            Operand<T> B, int V, int nm, int *info)
+                            Operand<T> B, int V, int nm, int *info, const int *mask)
 {
     cholgrid::CholGridArgs<T> args;
     args.a = A.p;
@@ -435,6 +437,7 @@ hipError_t launch_chol_grid(bool ldl, bool factor, bool up, int n, int nrhs, Ope
     args.nrhs = nrhs;
     args.info = info;
     args.factor = factor;
+    args.mask = mask;
     const unsigned blocks = team::xcd_grid((unsigned)((args.count + 3) / 4));
     if (ldl)
         hipLaunchKernelGGL((cholgrid::chol_grid_kernel<T, true>), dim3(blocks), dim3(256), 0,
@@ -446,8 +449,8 @@ hipError_t launch_chol_grid(bool ldl, bool factor, bool up, int n, int nrhs, Ope
 }
 
 template hipError_t launch_chol_grid<double>(bool, bool, bool, int, int, Operand<double>,
-                                             Operand<double>, int, int, int *);
+                                             Operand<double>, int, int, int *, const int *);
 template hipError_t launch_chol_grid<float>(bool, bool, bool, int, int, Operand<float>,
-                                            Operand<float>, int, int, int *);
+                                            Operand<float>, int, int, int *, const int *);
 
 } /* namespace cbk */

@@ -94,7 +94,7 @@ int dsposv_compact(char layout, char uplo, int n, int nrhs, double *ap, int ldap
                    size_t lwork, int *iter, int *info, int V, int nm);
 
 /* bytes of workspace dsposv_compact needs for this call (0 if none): 0 when
- * a fused kernel serves it (1 <= n <= 32, V >= 4, and leading
+ * a fused kernel serves it (1 <= n <= 64, V >= 4, and leading
  * dimensions small enough for the kernels' 32-bit offsets -- below about
  * 2^31 / (64 * 8 * n) at V = 64), else groups * V * (n*n + n*nrhs) * 4 */
 size_t dsposv_compact_worksize(char layout, char uplo, int n, int nrhs, int ldap, int ldbp,

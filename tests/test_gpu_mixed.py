@@ -17,9 +17,10 @@ Checks, per matrix:
     FLT_MAX; -3: a diagonal that is 0 in FP32, for sysvnp also a D whose FP32
     reciprocal overflows), equal to the oracle's, X then bit-identical to the
     oracle's FP64 solve.
-Both the fused kernels (n in 1..32, any nrhs in blocks; the quad kernel at
-17..24, the team kernel at the other orders, an order padded with the identity
-up to its kernel's: 24, or 16 / 24 / 32) and the
+Both the fused kernels (n in 1..64, any nrhs in blocks; the quad kernel at
+17..24, the team kernel at the other orders up to 32, an order padded with the
+identity up to its kernel's: 24, or 16 / 24 / 32; the row-per-lane kernel at
+33..64, its failures on the FP64 grid kernel) and the
 workspace lane kernel (other shapes, or the lane policy) are exercised.
 """
 
@@ -90,6 +91,10 @@ def cond1(A):
     (28, 4, 64, "L", "C", False),
     (12, 3, 8, "U", "R", False),
     (20, 5, 8, "L", "C", True),     # the same order through the lane kernel
+    (40, 4, 64, "L", "C", False),   # 33..64: one matrix per wavefront, a row per lane
+    (64, 3, 8, "U", "R", False),
+    (33, 9, 16, "L", "C", False),
+    (48, 1, 4, "U", "C", False),
     (7, 2, 2, "U", "R", False),     # padded up to 16
     (7, 2, 2, "U", "R", True),
     (1, 3, 4, "L", "C", False),
@@ -130,7 +135,8 @@ def test_dsposv_refines_to_fp64_accuracy(n, nrhs, V, uplo, layout, lane):
 
 
 @pytest.mark.parametrize("n,V,lane", [(32, 64, False), (32, 64, True), (24, 8, False),
-                                      (28, 8, False), (14, 8, False), (20, 8, False)])
+                                      (28, 8, False), (14, 8, False), (20, 8, False),
+                                      (40, 8, False), (64, 64, False)])
 def test_dsposv_fallback_and_info(n, V, lane):
     import cbk_amd
     r = cbkt.rng(7 + n)
@@ -184,6 +190,8 @@ def sym_indefinite_logspace(r, nm, n, logk):
     (24, 24, 64, "L", "C", True),   # the same through the workspace lane kernel
     (20, 5, 8, "L", "C", False),    # padded with the identity up to 24
     (30, 4, 16, "U", "R", False),   # up to 32
+    (40, 4, 64, "L", "C", False),   # 33..64: a row per lane
+    (64, 5, 8, "U", "R", False),
     (20, 5, 8, "L", "C", True),     # the lane kernel
     (24, 6, 32, "U", "C", False),
 ])
@@ -217,7 +225,7 @@ def test_dssysvnp_refines_to_fp64_accuracy(n, nrhs, V, uplo, layout, lane):
 
 
 @pytest.mark.parametrize("n,V,lane", [(24, 64, False), (24, 64, True), (32, 8, False),
-                                      (21, 8, False), (27, 8, False)])
+                                      (21, 8, False), (27, 8, False), (48, 8, False)])
 def test_dssysvnp_fallback_and_info(n, V, lane):
     import cbk_amd
     r = cbkt.rng(17 + n)
@@ -280,7 +288,8 @@ def iter_code_cases(n, ldl):
 @pytest.mark.parametrize("name", ["dsposv_compact", "dssysvnp_compact"])
 @pytest.mark.parametrize("n,V,lane", [(16, 8, False), (32, 4, False), (6, 8, False),
                                       (6, 8, True), (16, 8, True), (11, 8, False),
-                                      (19, 8, False), (29, 4, False)])
+                                      (19, 8, False), (29, 4, False), (40, 8, False),
+                                      (64, 4, False)])
 def test_mixed_iter_codes_match_oracle(name, n, V, lane):
     import cbk_amd
     ldl = name == "dssysvnp_compact"

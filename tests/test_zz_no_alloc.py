@@ -54,7 +54,7 @@ def _same(a, b):
 
 @pytest.mark.parametrize("shape", [
     # (n, nrhs, V, nm) of the first call, then of the smaller second call
-    ((40, 5, 8, 601), (40, 3, 8, 117)),    # the lane path (n > 32): workspace from the static array
+    ((72, 5, 8, 601), (72, 3, 8, 117)),    # the lane path (n > 64): workspace from the static array
     ((7, 2, 2, 999), (5, 1, 2, 33)),       # the lane path (V < 4)
     ((32, 4, 64, 300), (32, 2, 64, 70)),   # the fused kernel: no workspace at all
 ])
@@ -80,14 +80,14 @@ def test_noalloc_limit_and_codes():
     import cbk_amd
     z, fake = None, 8  # fake: a non-null address that is never dereferenced
     f = lambda *a: cbk_amd.call_ext("dsposv_compact_noalloc", *a)
-    # an order the lane kernel serves (n > 32) needs groups * V * (n*n + n*nrhs)
-    # * 4 bytes of workspace, here > 32 MiB: refused before any launch (the
-    # fused kernel's orders need none, so they would launch)
-    assert cbk_amd.dsposv_worksize("C", "L", 40, 5, 40, 40, 40, 8, 20000) > (32 << 20)
-    assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, fake, fake, 8, 20000) == CBK_ERR_SCRATCH
-    assert f("X", "L", 40, 5, z, 40, z, 40, z, 40, fake, fake, 8, 20000) == -1
-    assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, z, fake, 8, 10) == -11
-    assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, fake, z, 8, 10) == -12
+    # a call the lane kernel serves (V < 4; the fused kernels need none and
+    # would launch) needs groups * V * (n*n + n*nrhs) * 4 bytes of workspace,
+    # here > 32 MiB: refused before any launch
+    assert cbk_amd.dsposv_worksize("C", "L", 40, 5, 40, 40, 40, 2, 20000) > (32 << 20)
+    assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, fake, fake, 2, 20000) == CBK_ERR_SCRATCH
+    assert f("X", "L", 40, 5, z, 40, z, 40, z, 40, fake, fake, 2, 20000) == -1
+    assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, z, fake, 2, 10) == -11
+    assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, fake, z, 2, 10) == -12
     assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, fake, fake, 3, 10) == -13
-    assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, fake, fake, 8, -1) == -14
-    assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, z, z, 8, 0) == 0
+    assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, fake, fake, 2, -1) == -14
+    assert f("C", "L", 40, 5, z, 40, z, 40, z, 40, z, z, 2, 0) == 0
