@@ -94,15 +94,20 @@ template <class T> __device__ __forceinline__ T opaque(T x)
 template <bool LDL>
 __device__ __forceinline__ void solve32(float (&w)[Q], float dinv, const float *Lw, int lane, int n)
 {
-    /* forward: y_k (Cholesky: w_k / L(k, k)) from lane k, then the rows below */
-    sfor<0, NB>([&](auto kc) {
-        constexpr int k = kc;
-        if (k < n) {
-            const float lik = Lw[lane * LS + k];
-            const float dk = LDL ? 1.0f : lanef<k>(dinv);
-            sfor<0, Q>([&](auto q) {
-                const float yk = LDL ? lanef<k>(w[q]) : lanef<k>(w[q]) * dk;
-                w[q] = __builtin_fmaf(-lik, yk, w[q]);
+    /* forward: y_k (Cholesky: w_k / L(k, k)) from lane k, then the rows
+     * below; in chunks of 8 steps (a step past n changes nothing: its L
+     * entries and its lane's w are zero), so that a chunk's LDS reads are
+     * issued together */
+    sfor<0, NB / 8>([&](auto cc) {
+        if (8 * cc < n) {
+            sfor<8 * cc, 8 * cc + 8>([&](auto kc) {
+                constexpr int k = kc;
+                const float lik = Lw[lane * LS + k];
+                const float dk = LDL ? 1.0f : lanef<k>(dinv);
+                sfor<0, Q>([&](auto q) {
+                    const float yk = LDL ? lanef<k>(w[q]) : lanef<k>(w[q]) * dk;
+                    w[q] = __builtin_fmaf(-lik, yk, w[q]);
+                });
             });
         }
     });
@@ -110,14 +115,16 @@ __device__ __forceinline__ void solve32(float (&w)[Q], float dinv, const float *
     sfor<0, Q>([&](auto q) { w[q] = w[q] * dinv; });
     /* backward: x_i (Cholesky: z_i / L(i, i)) from lane i, then the rows
      * above with L(i, k) from row i */
-    sfor_down<0, NB>([&](auto ic) {
-        constexpr int i = ic;
-        if (i < n) {
-            const float lik = Lw[i * LS + lane];
-            const float di = LDL ? 1.0f : lanef<i>(dinv);
-            sfor<0, Q>([&](auto q) {
-                const float xi = LDL ? lanef<i>(w[q]) : lanef<i>(w[q]) * di;
-                w[q] = __builtin_fmaf(-lik, xi, w[q]);
+    sfor_down<0, NB / 8>([&](auto cc) {
+        if (8 * cc < n) {
+            sfor_down<8 * cc, 8 * cc + 8>([&](auto ic) {
+                constexpr int i = ic;
+                const float lik = Lw[i * LS + lane];
+                const float di = LDL ? 1.0f : lanef<i>(dinv);
+                sfor<0, Q>([&](auto q) {
+                    const float xi = LDL ? lanef<i>(w[q]) : lanef<i>(w[q]) * di;
+                    w[q] = __builtin_fmaf(-lik, xi, w[q]);
+                });
             });
         }
     });
@@ -275,9 +282,11 @@ mixed_rows_kernel(RowsArgs p)
             quad::lds_sync();
             double r[Q];
             sfor<0, Q>([&](auto q) { r[q] = b64[q]; });
-            sfor<0, NB>([&](auto j) {
-                if (j < n) {
-                    sfor<0, Q>([&](auto q) { r[q] = __builtin_fma(-a64[j], xw[j * Q + q], r[q]); });
+            sfor<0, NB / 8>([&](auto cc) {
+                if (8 * cc < n) { /* entries past n are zero */
+                    sfor<8 * cc, 8 * cc + 8>([&](auto j) {
+                        sfor<0, Q>([&](auto q) { r[q] = __builtin_fma(-a64[j], xw[j * Q + q], r[q]); });
+                    });
                 }
             });
             quad::lds_sync();
