@@ -191,16 +191,21 @@ __global__ __launch_bounds__(64) void mixed_refine_kernel(MixedRefineArgs p)
         int steps = 0;
         unsigned long long done_mask = 0; /* columns settled, bit q */
         for (int it = 0; it <= p.itermax; ++it) {
-            /* r = b - A x, lane = row, columns ascending */
+            /* r = b - A x, lane = row, columns ascending; guarded by chunks of 8
+             * columns (A and x are zero past n), so that a chunk's LDS reads
+             * issue together */
             if (lane < m) {
                 double r[Q];
 #pragma unroll
                 for (int q = 0; q < Q; ++q) r[q] = bq[q];
 #pragma unroll
-                for (int c = 0; c < NB; ++c) {
-                    if (c < n) {
+                for (int c8 = 0; c8 < NB; c8 += 8) {
+                    if (c8 < n) {
 #pragma unroll
-                        for (int q = 0; q < Q; ++q) r[q] = __builtin_fma(-sX[c * Q + q], arow[c], r[q]);
+                        for (int c = c8; c < c8 + 8; ++c) {
+#pragma unroll
+                            for (int q = 0; q < Q; ++q) r[q] = __builtin_fma(-sX[c * Q + q], arow[c], r[q]);
+                        }
                     }
                 }
 #pragma unroll
@@ -221,16 +226,23 @@ __global__ __launch_bounds__(64) void mixed_refine_kernel(MixedRefineArgs p)
 #pragma unroll
                 for (int q = 0; q < Q; ++q) sV[lane * Q + q] = (float)g[q];
             }
+            else if (lane < NB) {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) sV[lane * Q + q] = 0.0f; /* the chunks below read past n */
+            }
             __syncthreads();
             /* y = R^-T fl32(g): y_j = sum_{k <= j} R^-1(k, j) g_k, k ascending */
             float y[Q];
 #pragma unroll
             for (int q = 0; q < Q; ++q) y[q] = 0.0f;
 #pragma unroll
-            for (int k = 0; k < NB; ++k) {
-                if (k < n) {
+            for (int k8 = 0; k8 < NB; k8 += 8) {
+                if (k8 < n) { /* R^-1 and g are zero past n */
 #pragma unroll
-                    for (int q = 0; q < Q; ++q) y[q] = __builtin_fmaf(rcol[k], sV[k * Q + q], y[q]);
+                    for (int k = k8; k < k8 + 8; ++k) {
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) y[q] = __builtin_fmaf(rcol[k], sV[k * Q + q], y[q]);
+                    }
                 }
             }
             __syncthreads(); /* every lane has read g */
@@ -243,10 +255,13 @@ __global__ __launch_bounds__(64) void mixed_refine_kernel(MixedRefineArgs p)
 #pragma unroll
             for (int q = 0; q < Q; ++q) h[q] = 0.0f;
 #pragma unroll
-            for (int k = 0; k < NB; ++k) {
-                if (k < n) {
+            for (int k8 = 0; k8 < NB; k8 += 8) {
+                if (k8 < n) { /* R^-1 and y are zero past n */
 #pragma unroll
-                    for (int q = 0; q < Q; ++q) h[q] = __builtin_fmaf(rrow[k], sV[k * Q + q], h[q]);
+                    for (int k = k8; k < k8 + 8; ++k) {
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) h[q] = __builtin_fmaf(rrow[k], sV[k * Q + q], h[q]);
+                    }
                 }
             }
             /* x += h on the open columns; |h|, |x| to LDS */
