@@ -230,17 +230,23 @@ chol_grid_kernel(CholGridArgs<T> p)
         });
         /* forward: L y = b; step l: y_l = b_l (/ L(l, l)) from its row lane,
          * then the rows below */
-        sfor<0, NMAX>([&](auto lc) {
-            constexpr int l = lc;
-            constexpr int ll = l % 8, lb = l / 8;
-            if (l < n) {
-                const T y = rowbc<ll>(LDL ? bb[lb] : bb[lb] * dinv[lb]);
-                sfor<lb, S>([&](auto a) {
-                    const T lil = bperm(f[a][lb], raddr + 32 * ll); /* L(r + 8a, l) */
-                    if constexpr (a > lb)
-                        bb[a] = fmaT(-lil, y, bb[a]);
-                    else
-                        bb[a] = (r > ll) ? fmaT(-lil, y, bb[a]) : bb[a];
+        /* by slots of 8 steps: a step past n is an exact no-op (its y, and its
+         * L entries, are +0), so only whole slots are guarded and a slot's
+         * steps schedule together */
+        sfor<0, S>([&](auto lbc) {
+            constexpr int lb = lbc;
+            if (8 * lb < n) {
+                sfor<8 * lb, 8 * lb + 8>([&](auto lc) {
+                    constexpr int l = lc;
+                    constexpr int ll = l % 8;
+                    const T y = rowbc<ll>(LDL ? bb[lb] : bb[lb] * dinv[lb]);
+                    sfor<lb, S>([&](auto a) {
+                        const T lil = bperm(f[a][lb], raddr + 32 * ll); /* L(r + 8a, l) */
+                        if constexpr (a > lb)
+                            bb[a] = fmaT(-lil, y, bb[a]);
+                        else
+                            bb[a] = (r > ll) ? fmaT(-lil, y, bb[a]) : bb[a];
+                    });
                 });
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -249,17 +255,20 @@ chol_grid_kernel(CholGridArgs<T> p)
         sfor<0, S>([&](auto a) { bb[a] = bb[a] * dinv[a]; });
         /* backward: L^T x = y from the bottom; step l: x_l from its row lane,
          * then the rows above with L(l, i) from lane (l % 8, r) */
-        sfor_down<0, NMAX>([&](auto lc) {
-            constexpr int l = lc;
-            constexpr int ll = l % 8, lb = l / 8;
-            if (l < n) {
-                const T x = rowbc<ll>(LDL ? bb[lb] : bb[lb] * dinv[lb]);
-                sfor<0, lb + 1>([&](auto a) {
-                    const T cli = bperm(f[lb][a], 4 * ll + 32 * r); /* L(l, r + 8a) */
-                    if constexpr (a < lb)
-                        bb[a] = fmaT(-cli, x, bb[a]);
-                    else
-                        bb[a] = (r < ll) ? fmaT(-cli, x, bb[a]) : bb[a];
+        sfor_down<0, S>([&](auto lbc) {
+            constexpr int lb = lbc;
+            if (8 * lb < n) {
+                sfor_down<8 * lb, 8 * lb + 8>([&](auto lc) {
+                    constexpr int l = lc;
+                    constexpr int ll = l % 8;
+                    const T x = rowbc<ll>(LDL ? bb[lb] : bb[lb] * dinv[lb]);
+                    sfor<0, lb + 1>([&](auto a) {
+                        const T cli = bperm(f[lb][a], 4 * ll + 32 * r); /* L(l, r + 8a) */
+                        if constexpr (a < lb)
+                            bb[a] = fmaT(-cli, x, bb[a]);
+                        else
+                            bb[a] = (r < ll) ? fmaT(-cli, x, bb[a]) : bb[a];
+                    });
                 });
             }
             __builtin_amdgcn_sched_barrier(0);
