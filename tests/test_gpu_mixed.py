@@ -317,3 +317,52 @@ def test_mixed_iter_codes_match_oracle(name, n, V, lane):
         # the FP64 path ran: factor bit-identical, X within the solve's rounding
         assert np.array_equal(np.tril(Ag[m]), np.tril(Ao[m])), m
         assert np.allclose(X[m], Xo[m], rtol=1e-12, atol=1e-12 * np.abs(Xo[m]).max()), m
+
+
+@pytest.mark.parametrize("name", ["dsposv_compact", "dssysvnp_compact"])
+@pytest.mark.parametrize("n,nrhs,V,uplo,layout", [(20, 3, 8, "L", "C"), (30, 5, 16, "U", "C"),
+                                                  (40, 4, 8, "L", "C"), (64, 2, 4, "U", "R")])
+def test_mixed_padded_leading_dimensions(name, n, nrhs, V, uplo, layout):
+    """ldap, ldbp, ldxp beyond the order: the same X as the oracle's to its
+    tolerance, A untouched on success, and the padding of A and X untouched
+    (the fused kernels address rows and columns through the strides)"""
+    import cbk_amd
+    import gpuutil
+    torch = gpuutil.torch()
+    ldl = name == "dssysvnp_compact"
+    r = cbkt.rng(700 + n + nrhs)
+    nm = 2 * V + 3
+    A = sym_indefinite_logspace(r, nm, n, 3) if ldl else spd_logspace(r, nm, n)
+    B = r.standard_normal((nm, n, nrhs))
+    row = layout == "R"
+    lda = n + 3
+    ldb = (nrhs if row else n) + 2
+    Aref = np.where(np.eye(n, dtype=bool), A, np.tril(A) if uplo == "L" else np.triu(A))
+    ap = cbkt.pack(Aref, V, ld=lda, row=row)
+    bp = cbkt.pack(B, V, ld=ldb, row=row, pad_diag=0.0)
+    xp = np.full_like(bp, cbkt.SENTINEL)
+    print(f"case {name} padded ld n={n} nrhs={nrhs} V={V} {layout}{uplo}", flush=True)
+    dA, dB, dX = (gpuutil.to_dev(x) for x in (ap, bp, xp))
+    dI = gpuutil.to_dev(np.zeros(nm, np.int32))
+    dN = gpuutil.to_dev(np.zeros(nm, np.int32))
+    ws = cbk_amd.dsposv_worksize(layout, uplo, n, nrhs, lda, ldb, ldb, V, nm)
+    dW = torch.empty(max(ws, 8), dtype=torch.uint8, device="cuda") if ws else None
+    st = cbk_amd.call_ext(name, layout, uplo, n, nrhs, dA, lda, dB, ldb, dX, ldb, dW, ws, dI, dN, V, nm)
+    assert cbk_amd.synchronize() == 0, cbk_amd.last_error()
+    assert st == 0
+    ag, xg, it, info = dA.cpu().numpy(), dX.cpu().numpy(), dI.cpu().numpy(), dN.cpu().numpy()
+    assert np.all(info == 0) and np.all(it >= 0), (it, info)
+    assert np.array_equal(ag, ap)                       # A (and its padding) untouched
+    X = cbkt.unpack(xg, nm, n, nrhs, V, ld=ldb, row=row)
+    # X's padding rows / columns keep their sentinel
+    Xall = cbkt.unpack(xg, nm, ldb if not row else n, nrhs if not row else ldb, V, ld=ldb, row=row)
+    if row:
+        assert np.all(Xall[:, :, nrhs:] == cbkt.SENTINEL)
+    else:
+        assert np.all(Xall[:, n:, :] == cbkt.SENTINEL)
+    so, _, xo, io, no = run_dsposv("oracle", A, B, V, uplo, layout, name=name)
+    assert so == 0 and np.all(io >= 0)
+    Xo = cbkt.unpack(xo, nm, n, nrhs, V, row=row)
+    tol = 4 * n * 2.0 ** -53 * cond1(A)
+    ratio = np.abs(X - Xo).max(axis=1) / np.abs(Xo).max(axis=1) / tol[:, None]
+    assert np.all(ratio <= 1.0), ratio.max()
