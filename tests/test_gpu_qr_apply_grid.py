@@ -21,7 +21,7 @@ def _gpu():
     return gpuutil
 
 
-RUNS = [("C", 64, 130, 1), ("R", 32, 71, 0), ("C", 8, 13, 1), ("R", 4, 6, 1)]
+RUNS = [("C", 64, 130, 1), ("R", 32, 71, 0), ("C", 8, 13, 1), ("R", 4, 6, 1), ("C", 2, 5, 0)]
 
 
 @pytest.mark.parametrize("p", ["d", "s"])
